@@ -1,0 +1,210 @@
+"""Benchmark: all-pairs regularised Biot-Savart, velocity + stretching.
+
+Workload: random vortex-blob cloud, N = 1,048,576 particles (positions
+U[0,2pi]^3, strengths N(0,1/N), sigma = 1.3*2pi/N^(1/3), seed 3+log2 N);
+one step = one fused FP32-compensated velocity + stretching evaluation over
+all N^2 pairs (Morton sort + pack + pair pass), inputs resident in HBM.
+
+Prints one JSON line: pairs/s, the fused kernel's roofline position
+(algorithmic FLOP/s against the MI355X FP32 vector peak, from HIP events on
+the launch stream), rel-L2 of the FP32 result against FP64 on a strided target
+sample, the plain FP64 drop-in kernels' time on the same input, the adjoint /
+forward time ratio and the CPU baseline (C/OpenMP FP64 port of the reference's
+Numba CPU plugin) timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+       [--skip-cpu] [--skip-plain] [--skip-adjoint] [--json-only]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "pair-interactions/s & % FP32 peak at N=1M (vs FP64 plain); adjoint/fwd ratio"
+FP32_VECTOR_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (VALU) peak
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+# Algorithmic flops per pair of the fused velocity + stretching pass with a
+# uniform radius (shared r, rho^2, one rsqrt, t^5/t^7 powers, the velocity
+# cross product, the alpha_j sum for alpha_i x G and the f2 (alpha_i.c) r
+# term); Kahan and FP64 folds are not counted.
+FLOPS_PER_PAIR_VEL_STRETCH = 53
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--cpu-targets", type=int, default=8192)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the pair sum shards by targets; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from openonda_amd import flow_models as FM
+    from openonda_amd import physics as P
+    from openonda_amd import _lib as L
+    import ctypes as C
+
+    P.require_gpu()
+    dev = torch.device("cuda", 0)
+    n = args.n
+    log(f"[bench] building random cloud N={n}")
+    x, a, s, nu = FM.random_cloud(n)
+    pos, alpha = P.to_soa(x, dev), P.to_soa(a, dev)
+    sig = torch.from_numpy(s).to(dev)
+    nu_t = torch.from_numpy(nu).to(dev)
+
+    outputs = ("vel", "stretch")
+    opts = L.ForwardOpts(L.OUT_VEL | L.OUT_STRETCH, L.PREC_F32C, L.SIGMA_UNIFORM, 0,
+                         args.tile_config)
+    part = P._particles(pos, alpha, sig)
+    vel = torch.empty((3, n), dtype=torch.float64, device=dev)
+    st = torch.empty((3, n), dtype=torch.float64, device=dev)
+    fo = L.ForwardOut(n, vel.data_ptr(), st.data_ptr(), None, None, None)
+    lib = L.load()
+    wsb = lib.vpm_forward_workspace_bytes(n, C.byref(opts))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    def step(ev=None):
+        L.call("vpm_forward_prepare", C.byref(part), C.byref(opts), ws.data_ptr(), wsb, sh)
+        if ev is not None:
+            ev[0].record(stream)
+        L.call("vpm_forward_run", C.byref(part), C.byref(opts), C.byref(fo), ws.data_ptr(), wsb,
+               sh)
+        if ev is not None:
+            ev[1].record(stream)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    log(f"[bench] timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(events[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    kern_ms = sum(e0.elapsed_time(e1) for e0, e1 in events) / args.steps
+    pairs = float(n) * float(n)
+    value = pairs / (ms_step * 1e-3)
+    achieved = FLOPS_PER_PAIR_VEL_STRETCH * pairs / (kern_ms * 1e-3) / 1e12
+    log(f"[bench] f32c: {ms_step:.2f} ms/step, kernel {kern_ms:.2f} ms, {value:.3e} pairs/s, "
+        f"{achieved:.1f} TFLOP/s")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"random vortex-blob cloud, N={n}, fused velocity+stretching, "
+                        "FP32 pairs with compensated FP64 totals",
+            "n": n, "outputs": list(outputs), "precision": "f32c",
+            "sigma": float(s[0]), "seed": 3 + int(round(math.log2(n))),
+            "tile_config": args.tile_config,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "fused_kernel<float, VEL|STRETCH, uniform sigma>",
+            "kernel_ms": kern_ms,
+            "flops_per_pair": FLOPS_PER_PAIR_VEL_STRETCH,
+        },
+    }
+
+    # ---- plain FP64 drop-in kernels on the same input ---------------------
+    if not args.skip_plain:
+        log("[bench] plain FP64 velocity + stretching")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        P.selfinduced_velocity(pos, alpha, sig)
+        P.strength_gradients(pos, alpha, sig)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1)
+        result["plain_f64"] = {"ms": plain_ms, "pairs_per_s": pairs / (plain_ms * 1e-3),
+                               "speedup_f32c_vs_plain": plain_ms / ms_step}
+        log(f"[bench] plain f64: {plain_ms:.1f} ms ({plain_ms / ms_step:.2f}x slower)")
+        # fused FP64 (same sharing of r, rho^2, one rsqrt per pair)
+        o64 = L.ForwardOpts(L.OUT_VEL | L.OUT_STRETCH, L.PREC_F64, L.SIGMA_UNIFORM, 0, -1)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        P.forward(pos, alpha, sig, outputs=outputs, precision="f64", sigma_mode="uniform")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        f64_ms = e0.elapsed_time(e1)
+        result["fused_f64"] = {"ms": f64_ms, "speedup_f32c_vs_fused_f64": f64_ms / ms_step,
+                               "fp64_tflops": FLOPS_PER_PAIR_VEL_STRETCH * pairs / f64_ms / 1e9,
+                               "fp64_peak_tflops": FP64_VECTOR_PEAK_TFLOPS}
+        log(f"[bench] fused f64: {f64_ms:.1f} ms")
+
+    # ---- CPU baseline + FP64 accuracy sample -------------------------------
+    if not args.skip_cpu:
+        from oracle import cpu_baseline as CB
+
+        m = min(args.cpu_targets, n)
+        log(f"[bench] CPU baseline: {m} strided targets x {n} sources, {CB.threads()} threads")
+        pps, secs, (u_ref, S_ref), tidx = CB.time_velocity_stretching(x, a, s, m)
+        u = vel[:, torch.from_numpy(tidx).to(dev)].t().cpu().numpy()
+        S = st[:, torch.from_numpy(tidx).to(dev)].t().cpu().numpy()
+        rel = {"vel": float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)),
+               "stretch": float(np.linalg.norm(S - S_ref) / np.linalg.norm(S_ref))}
+        result["rel_l2_vs_fp64"] = rel
+        result["rel_l2_sample"] = f"{len(tidx)} strided targets (stride {n // m}) vs all sources"
+        result["cpu_baseline"] = {
+            "value": pps, "unit": "pair-interactions/s", "cores": CB.threads(), "kind": "port",
+            "sample": f"velocity + stretching (two passes, as the reference runs them) for "
+                      f"{len(tidx)} strided targets x {n} sources in {secs:.1f} s; "
+                      f"C/OpenMP FP64 port of ParticlesPhysics.py, -O3 -ffast-math",
+            "cpu_model": CB.host_cpu_model(),
+        }
+        log(f"[bench] CPU {pps:.3e} pairs/s ({secs:.1f} s); rel-L2 {rel}")
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,185 @@
+/*
+ * vpm_hip.h -- C ABI of the MI355X (gfx950) all-pairs regularised Biot-Savart
+ * library (libvpm_hip.so).
+ *
+ * The reference solver's hot path is a duck-typed Python "physics module"
+ * chosen by importlib (OpenONDA/solvers/VPM/vpmModule.py:141-154) whose eight
+ * functions take host float64 arrays of shape (N,3)/(N,) and return fresh
+ * host arrays (OpenONDA/solvers/VPM/ParticlesPhysicsCuda.py:11,103,198,351,
+ * 452,537,647,756).  This header is the native boundary that replaces the
+ * Numba-CUDA kernels behind those functions.  A ctypes binding
+ * (openonda_amd/_lib.py) drives it with PyTorch-ROCm device buffers.
+ *
+ * Conventions
+ *  - Every pointer is a device pointer unless stated otherwise.  The caller
+ *    owns every buffer; the library allocates nothing persistent except the
+ *    HIP graph objects of vpm_rk4_graph_* and kernel-resident constants.
+ *  - Vector fields are SoA: component c of particle i lives at p[c*ld + i]
+ *    (ld >= n).  Scalars per particle are dense arrays of length n.
+ *  - All work is enqueued on the given hipStream_t (passed as void*, NULL =
+ *    the default stream).  No call synchronises the device except where noted.
+ *  - Return value 0 = success, < 0 = error (see VPM_ERR_*).  The message of
+ *    the last error of the calling thread is returned by vpm_last_error().
+ *  - No C++ exception crosses this boundary.
+ */
+#ifndef VPM_HIP_H
+#define VPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPM_ABI_VERSION 1
+
+#define VPM_OK 0
+#define VPM_ERR_ARG (-1)        /* bad shape, stride, option or null pointer */
+#define VPM_ERR_DEVICE (-2)     /* no HIP device / pointer not on the device */
+#define VPM_ERR_LAUNCH (-3)     /* HIP runtime or launch failure */
+#define VPM_ERR_WORKSPACE (-4)  /* caller workspace too small */
+#define VPM_ERR_UNSUPPORTED (-5)
+
+/* Which reference plugin's arithmetic to reproduce.
+ * CUDA: OpenONDA/solvers/VPM/ParticlesPhysicsCuda.py (the GPU plugin).
+ * CPU : OpenONDA/solvers/VPM/ParticlesPhysics.py (the Numba CPU plugin:
+ *       sigma_i in the self-induced velocity, full-curl vorticity, PSE with
+ *       nu_ij and A+1e-8, KE with A+1e-8, single 1/(4pi) or 1/(8pi) on the
+ *       helicity and enstrophy). */
+#define VPM_VARIANT_CUDA 0
+#define VPM_VARIANT_CPU 1
+
+/* A particle set (sources, and targets for self-evaluations). */
+typedef struct vpm_particles {
+  int64_t n;           /* number of particles, >= 0 */
+  int64_t ld;          /* component stride of pos/str, >= n */
+  const double* pos;   /* [3][ld] positions x_i */
+  const double* str;   /* [3][ld] strengths alpha_i = omega_i vol_i */
+  const double* sigma; /* [n] smoothing radii */
+  const double* nu;    /* [n] effective viscosities, may be NULL unless PSE */
+} vpm_particles_t;
+
+/* Evaluation points for the *_at functions. */
+typedef struct vpm_points {
+  int64_t m;
+  int64_t ld;          /* component stride, >= m */
+  const double* pos;   /* [3][ld] */
+} vpm_points_t;
+
+/* ---- library ---------------------------------------------------------- */
+
+int vpm_abi_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* vpm_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int vpm_device_count(void);
+
+/* ---- plain FP64 drop-in: one entry point per reference function --------
+ * Faithful FP64 restatements of the reference arithmetic (IEEE division and
+ * pow), with an LDS-tiled SoA memory design.  Results agree with the FP64
+ * oracle to a few ulp of the sum. */
+
+/* ParticlesPhysicsCuda.py:11-97 (get_induced_velocity_at; same arithmetic in
+ * ParticlesPhysics.py:9-74).  out[3][ld_out] at the m targets. */
+int vpm_get_induced_velocity_at(const vpm_points_t* targets, const vpm_particles_t* src,
+                                double* out, int64_t ld_out, void* stream);
+
+/* ParticlesPhysicsCuda.py:103-191 (variant CUDA: sigma_ij) and
+ * ParticlesPhysics.py:81-143 (variant CPU: sigma_i). */
+int vpm_get_selfinduced_velocity(const vpm_particles_t* p, int variant,
+                                 double* out, int64_t ld_out, void* stream);
+
+/* ParticlesPhysicsCuda.py:198-219,291-346 (CUDA: simplified zeta form) and
+ * ParticlesPhysics.py:211-262 (CPU: full curl form).  Output has m rows. */
+int vpm_get_vorticity_field_at(const vpm_points_t* targets, const vpm_particles_t* src,
+                               int variant, double* out, int64_t ld_out, void* stream);
+
+/* ParticlesPhysicsCuda.py:351-445 and ParticlesPhysics.py:268-312 (same). */
+int vpm_get_strength_gradients(const vpm_particles_t* p, double* out, int64_t ld_out,
+                               void* stream);
+
+/* ParticlesPhysicsCuda.py:452-530 (CUDA: nu_i) and ParticlesPhysics.py:318-382
+ * (CPU: nu_ij, A + 1e-8).  p->nu must be non-NULL. */
+int vpm_get_strength_gradients_pse(const vpm_particles_t* p, int variant,
+                                   double* out, int64_t ld_out, void* stream);
+
+/* Scalar pair reductions.  *result (device, 1 double) is overwritten; the sum
+ * is deterministic (fixed-order two-pass).  workspace must hold
+ * vpm_reduction_workspace_bytes(n) bytes.
+ * ParticlesPhysicsCuda.py:537-642 / ParticlesPhysics.py:388-429 (energy),
+ * ParticlesPhysicsCuda.py:647-750 / ParticlesPhysics.py:432-468 (helicity),
+ * ParticlesPhysicsCuda.py:756-777,867-920 / ParticlesPhysics.py:517-550
+ * (enstrophy). */
+size_t vpm_reduction_workspace_bytes(int64_t n);
+int vpm_get_total_kinetic_energy(const vpm_particles_t* p, int variant, double* result,
+                                 void* workspace, size_t ws_bytes, void* stream);
+int vpm_get_total_helicity(const vpm_particles_t* p, int variant, double* result,
+                           void* workspace, size_t ws_bytes, void* stream);
+int vpm_get_total_enstrophy(const vpm_particles_t* p, int variant, double* result,
+                            void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- fused forward (the fast path) -------------------------------------
+ * One all-pairs pass computing any subset of the GPU-variant outputs
+ * (ParticlesPhysicsCuda.py semantics):
+ *   VEL      self-induced velocity      (:133-191)
+ *   STRETCH  transpose-scheme stretching (:375-445)
+ *   PSE      PSE diffusion with nu_i     (:478-530)
+ *   VORT     LES smoothed vorticity      (:291-346, targets = particles)
+ *   ENSTROPHY total enstrophy scalar     (:756-777,867-920)
+ * precision F32C: FP32 pair arithmetic on block-centred offsets of
+ * Morton-sorted particles, FP32 sub-tile partials combined with Kahan
+ * compensation per source tile and folded into FP64 totals.
+ * precision F64: the same pass in FP64 throughout. */
+#define VPM_OUT_VEL 1u
+#define VPM_OUT_STRETCH 2u
+#define VPM_OUT_PSE 4u
+#define VPM_OUT_VORT 8u
+#define VPM_OUT_ENSTROPHY 16u
+
+#define VPM_PREC_F64 0
+#define VPM_PREC_F32C 1
+
+#define VPM_SIGMA_GENERAL 0 /* sigma_ij = (sigma_i+sigma_j)/2 per pair */
+#define VPM_SIGMA_UNIFORM 1 /* caller asserts sigma_i == sigma[0] for all i */
+
+typedef struct vpm_forward_opts {
+  uint32_t outputs;     /* OR of VPM_OUT_* (non-zero) */
+  int32_t precision;    /* VPM_PREC_* */
+  int32_t sigma_mode;   /* VPM_SIGMA_* */
+  int32_t presorted;    /* 1: particles already spatially coherent, skip the Morton sort */
+  int32_t tile_config;  /* -1: default; else index < vpm_tile_config_count() */
+} vpm_forward_opts_t;
+
+typedef struct vpm_forward_out {
+  int64_t ld;           /* component stride of the vector outputs, >= n */
+  double* vel;          /* [3][ld] or NULL (required iff VEL requested) */
+  double* stretch;      /* [3][ld] or NULL */
+  double* pse;          /* [3][ld] or NULL */
+  double* vort;         /* [3][ld] or NULL */
+  double* enstrophy;    /* [1] or NULL */
+} vpm_forward_out_t;
+
+size_t vpm_forward_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts);
+/* vpm_forward = vpm_forward_prepare + vpm_forward_run on the same workspace.
+ * prepare sorts (unless presorted) and packs the particles into the
+ * workspace; run executes the pair pass and the reductions.  run may be
+ * repeated on one prepared workspace while the particles are unchanged. */
+int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
+                vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream);
+int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
+                        void* workspace, size_t ws_bytes, void* stream);
+int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
+                    vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream);
+
+/* Tile configurations the fused F32C kernel is instantiated for (used by the
+ * autotuner).  Writes block threads, targets per lane, source tile, sub-tile
+ * length and compensation mode (0 Kahan+FP64, 1 direct FP64 fold). */
+int vpm_tile_config_count(void);
+int vpm_tile_config_info(int index, int* block, int* targets_per_lane, int* tile,
+                         int* subtile, int* comp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VPM_HIP_H */

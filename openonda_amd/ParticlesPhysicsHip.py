@@ -1,0 +1,134 @@
+"""Drop-in physics module for the reference ParticleSystem (HIP, MI355X).
+
+Same eight functions, argument order, host float64 NumPy in / fresh NumPy out
+semantics as ``OpenONDA/solvers/VPM/ParticlesPhysicsCuda.py`` (the module the
+reference loads with ``importlib`` at ``vpmModule.py:141-154``), computed by
+the plain FP64 HIP kernels of libvpm_hip.so.  Each call copies its inputs to
+the device, runs one kernel and copies the result back, like the reference
+wrappers (``ParticlesPhysicsCuda.py:16-34``).
+
+Differences from the reference, all deliberate:
+ * ``get_vorticity_field_at`` returns one row per *target* (the reference
+   sizes grid and output by the source count, ``ParticlesPhysicsCuda.py:199-207``;
+   both agree whenever M == N, the only way ``vpmModule.py`` calls it).
+ * The scalar reductions start from zero and sum in a fixed order (the
+   reference accumulates with FP64 atomics into an uninitialised device
+   scalar, ``ParticlesPhysicsCuda.py:545,642``).
+ * Inputs that are not float64 raise ``TypeError`` and bad shapes raise
+   ``ValueError`` (the reference's explicit Numba signatures reject the
+   former, ``ParticlesPhysicsCuda.py:39``).
+
+``VARIANT`` selects whose arithmetic is reproduced: "cuda" (this module's
+default, the GPU plugin) or "cpu" (``ParticlesPhysics.py``, the Numba CPU
+plugin), both evaluated on the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import physics as P
+
+VARIANT = "cuda"
+
+
+def _arr(a, name, cols=3):
+    if not isinstance(a, np.ndarray):
+        a = np.asarray(a)
+    if a.dtype != np.float64:
+        raise TypeError(f"{name} must be a float64 array, got {a.dtype}")
+    if cols is None:
+        if a.ndim != 1:
+            raise ValueError(f"{name} must have shape (N,), got {a.shape}")
+    elif a.ndim != 2 or a.shape[1] != cols:
+        raise ValueError(f"{name} must have shape (N, {cols}), got {a.shape}")
+    return a
+
+
+def _dev():
+    P.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _soa(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+
+
+def _src(x, a, s, dev, nu=None):
+    x = _arr(x, "particles_positions")
+    a = _arr(a, "particles_strengths")
+    s = _arr(s, "particles_radii", None)
+    n = x.shape[0]
+    if a.shape[0] != n or s.shape[0] != n:
+        raise ValueError("positions, strengths and radii must have the same length")
+    out = [_soa(x, dev), _soa(a, dev), torch.from_numpy(np.ascontiguousarray(s)).to(dev)]
+    if nu is not None:
+        nu = _arr(nu, "viscosities", None)
+        if nu.shape[0] != n:
+            raise ValueError("viscosities must have the same length as positions")
+        out.append(torch.from_numpy(np.ascontiguousarray(nu)).to(dev))
+    return out
+
+
+def _host(t):
+    return P.from_soa(t)
+
+
+def get_induced_velocity_at(target_positions, particles_positions, particles_strengths,
+                            particles_radii):
+    """Velocity at targets, (M,3).  ParticlesPhysicsCuda.py:11-34."""
+    dev = _dev()
+    y = _soa(_arr(target_positions, "target_positions"), dev)
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    return _host(P.induced_velocity_at(y, x, a, s))
+
+
+def get_selfinduced_velocity(particles_positions, particles_strengths, particles_radii):
+    """Self-induced velocity, (N,3).  ParticlesPhysicsCuda.py:103-127."""
+    dev = _dev()
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    return _host(P.selfinduced_velocity(x, a, s, VARIANT))
+
+
+def get_vorticity_field_at(target_positions, particle_positions, particle_strengths,
+                           particle_radii):
+    """Smoothed vorticity at targets, (M,3).  ParticlesPhysicsCuda.py:198-219."""
+    dev = _dev()
+    y = _soa(_arr(target_positions, "target_positions"), dev)
+    x, a, s = _src(particle_positions, particle_strengths, particle_radii, dev)
+    return _host(P.vorticity_field_at(y, x, a, s, VARIANT))
+
+
+def get_strength_gradients(particle_positions, particle_strengths, particle_radii):
+    """Stretching dalpha/dt, (N,3).  ParticlesPhysicsCuda.py:351-370."""
+    dev = _dev()
+    x, a, s = _src(particle_positions, particle_strengths, particle_radii, dev)
+    return _host(P.strength_gradients(x, a, s))
+
+
+def get_strength_gradients_PSE(positions, strengths, radii, viscosities):
+    """PSE dalpha/dt, (N,3).  ParticlesPhysicsCuda.py:452-473."""
+    dev = _dev()
+    x, a, s, nu = _src(positions, strengths, radii, dev, viscosities)
+    return _host(P.strength_gradients_pse(x, a, s, nu, VARIANT))
+
+
+def get_total_kinetic_energy(particles_positions, particles_strengths, particles_radii):
+    """Kinetic energy (float).  ParticlesPhysicsCuda.py:537-558."""
+    dev = _dev()
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    return float(P.total_kinetic_energy(x, a, s, VARIANT).item())
+
+
+def get_total_helicity(particles_positions, particles_strengths, particles_radii):
+    """Helicity (float).  ParticlesPhysicsCuda.py:647-668."""
+    dev = _dev()
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    return float(P.total_helicity(x, a, s, VARIANT).item())
+
+
+def get_total_enstrophy(particles_positions, particles_strengths, particles_radii):
+    """Enstrophy (float).  ParticlesPhysicsCuda.py:756-777."""
+    dev = _dev()
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    return float(P.total_enstrophy(x, a, s, VARIANT).item())

@@ -1,0 +1,96 @@
+// Internal helpers shared by the translation units of libvpm_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "vpm_hip.h"
+
+namespace vpm {
+
+constexpr double kOneOverFourPi = 0.079577471545947667884441881686257181;  // 1/(4 pi)
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kEpsPair = 1e-8;  // ParticlesPhysicsCuda.py:6, ParticlesPhysics.py:4
+
+// Thread-local error message (vpm_last_error()).
+void set_error(const char* fmt, ...);
+void clear_error();
+
+inline int fail(int code, const char* what) {
+  set_error("%s", what);
+  return code;
+}
+
+#define VPM_HIP_TRY(expr)                                                       \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ::vpm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                       __FILE__, __LINE__);                                     \
+      return VPM_ERR_LAUNCH;                                                    \
+    }                                                                           \
+  } while (0)
+
+// Check the launch that was just enqueued.
+#define VPM_LAUNCH_CHECK(name)                                                  \
+  do {                                                                          \
+    hipError_t e_ = hipGetLastError();                                          \
+    if (e_ != hipSuccess) {                                                     \
+      ::vpm::set_error("launch of %s failed: %s", name, hipGetErrorString(e_)); \
+      return VPM_ERR_LAUNCH;                                                    \
+    }                                                                           \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Validates a particle set; `need_nu` requires the viscosity array.
+int check_particles(const vpm_particles_t* p, bool need_nu, const char* who);
+int check_points(const vpm_points_t* t, const char* who);
+int check_out(const double* out, int64_t ld, int64_t n, const char* who);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Wave64 butterfly reductions.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Deterministic block sum (fixed shuffle tree + fixed LDS order).  All
+// threads of the block must call it; the result is valid in thread 0.
+template <int BLOCK>
+__device__ inline double block_sum(double v, double* lds /* BLOCK/64 */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) s += lds[w];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Sums `n` doubles in fixed order with one block and writes scale*sum.
+__global__ void final_sum_kernel(const double* __restrict__ part, int64_t n, double scale,
+                                 double* __restrict__ out);
+
+}  // namespace vpm

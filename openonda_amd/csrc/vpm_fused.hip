@@ -1,0 +1,796 @@
+// Fused all-pairs forward pass (the fast path).
+//
+// One pass over all (target, source) pairs computes any subset of the GPU
+// plugin's outputs -- self-induced velocity, transpose-scheme stretching, PSE
+// diffusion, LES smoothed vorticity and the enstrophy scalar -- sharing r_ij,
+// A = rho^2 + 1 and a single rsqrt per pair (A^-5/2 = t^5, A^-7/2 = t^7,
+// A^-9/2 = t^9 with t = rsqrt(A)).  Reference arithmetic being fused:
+// ParticlesPhysicsCuda.py:159-191 (velocity), :396-440 (stretching), :500-525
+// (PSE), :319-346 (vorticity), :886-909 + :777 (enstrophy).
+//
+// Layout and precision:
+//  * Prepass: bounding box -> 30-bit Morton keys -> radix sort (hipCUB) ->
+//    gather into a packed 64-byte FP64 record per particle (x,y,z,a,s,nu).
+//    Sorting makes each block of targets spatially compact.
+//  * A workgroup owns BLOCK*R consecutive sorted targets (R per lane,
+//    register blocked).  It computes its bounding-box centre c and stages each
+//    source tile into LDS as FP32 offsets (x_j - c)*scale computed in FP64,
+//    so FP32 pair arithmetic works on small numbers: r_ij keeps ~24 bits
+//    relative to the block size instead of to |x|.  With uniform sigma the
+//    offsets are pre-scaled by 1/sigma, so r^2 is rho^2 directly.
+//  * Each lane reads the same LDS record (broadcast) and updates R targets.
+//  * Accumulation: FP32 partials over SUB sources; partials enter a Kahan
+//    (sum, compensation) pair per tile (COMP=0) and the compensated tile sum
+//    is folded into FP64 totals; COMP=1 folds partials straight into FP64.
+//  * Epilogue scales by the 1/(4 pi) and 1/sigma^k constants in FP64 and
+//    scatters to the caller's (unsorted) order.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "vpm_common.h"
+
+namespace vpm {
+
+namespace {
+
+struct alignas(16) Src8 {
+  double x, y, z, ax, ay, az, s, nu;
+};
+
+// ------------------------------------------------------------------ prepass
+constexpr int kPreBlock = 256;
+constexpr int kBboxBlocks = 512;
+
+__global__ __launch_bounds__(kPreBlock) void bbox_partial_kernel(const double* __restrict__ pos,
+                                                                 int64_t ld, int n,
+                                                                 double* __restrict__ part) {
+  __shared__ double red[6][kPreBlock / 64];
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = blockIdx.x * kPreBlock + threadIdx.x; i < n; i += gridDim.x * kPreBlock) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double v = pos[c * ld + i];
+      mn[c] = fmin(mn[c], v);
+      mx[c] = fmax(mx[c], v);
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = wave_min(mn[c]);
+    mx[c] = wave_max(mx[c]);
+    if (lane == 0) {
+      red[c][wave] = mn[c];
+      red[3 + c][wave] = mx[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = red[threadIdx.x][0];
+    for (int w = 1; w < kPreBlock / 64; ++w)
+      v = threadIdx.x < 3 ? fmin(v, red[threadIdx.x][w]) : fmax(v, red[threadIdx.x][w]);
+    part[blockIdx.x * 6 + threadIdx.x] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void bbox_final_kernel(const double* __restrict__ part, int nb,
+                                                        double* __restrict__ bbox) {
+  if (threadIdx.x < 6) {
+    double v = part[threadIdx.x];
+    for (int b = 1; b < nb; ++b)
+      v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x]) : fmax(v, part[b * 6 + threadIdx.x]);
+    bbox[threadIdx.x] = v;
+  }
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(kPreBlock) void morton_kernel(const double* __restrict__ pos,
+                                                           int64_t ld, int n,
+                                                           const double* __restrict__ bbox,
+                                                           uint32_t* __restrict__ keys,
+                                                           int* __restrict__ idx) {
+  const int i = blockIdx.x * kPreBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t q[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double lo = bbox[c], ext = bbox[3 + c] - lo;
+    const double f = ext > 0.0 ? (pos[c * ld + i] - lo) / ext : 0.0;
+    int v = static_cast<int>(f * 1023.0);
+    v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+    q[c] = static_cast<uint32_t>(v);
+  }
+  keys[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+  idx[i] = i;
+}
+
+__global__ __launch_bounds__(kPreBlock) void gather_kernel(const vpm_particles_t p, int n,
+                                                           const int* __restrict__ perm,
+                                                           Src8* __restrict__ out) {
+  const int k = blockIdx.x * kPreBlock + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm ? perm[k] : k;
+  Src8 r;
+  r.x = p.pos[i];
+  r.y = p.pos[p.ld + i];
+  r.z = p.pos[2 * p.ld + i];
+  r.ax = p.str[i];
+  r.ay = p.str[p.ld + i];
+  r.az = p.str[2 * p.ld + i];
+  r.s = p.sigma[i];
+  r.nu = p.nu ? p.nu[i] : 0.0;
+  out[k] = r;
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ main pass
+struct FusedArgs {
+  int n;
+  const Src8* src;      // sorted particles
+  const int* perm;      // sorted k -> caller index (null: identity)
+  double* vel;          // outputs (caller order); unrequested ones point to scratch
+  double* str;
+  double* pse;
+  double* vort;
+  int64_t ld_vel, ld_str, ld_pse, ld_vort;
+  double* ens_part;     // per-block enstrophy partials
+};
+
+template <typename Real>
+struct Arith;
+template <>
+struct Arith<float> {
+  static __device__ inline float rsq(float a) { return __builtin_amdgcn_rsqf(a); }
+  static __device__ inline float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
+};
+template <>
+struct Arith<double> {
+  static __device__ inline double rsq(double a) { return 1.0 / sqrt(a); }
+  static __device__ inline double rcp(double a) { return 1.0 / a; }
+};
+
+// LDS source record.  Uniform sigma needs position + strength; general sigma
+// adds sigma_j/2, 1/sigma_j^2 and 1/sigma_j^3.
+template <typename Real, bool UNIF>
+struct alignas(16) SrcR;
+template <typename Real>
+struct alignas(16) SrcR<Real, true> {
+  Real x, y, z, ax, ay, az, p0, p1;
+};
+template <typename Real>
+struct alignas(16) SrcR<Real, false> {
+  Real x, y, z, ax, ay, az, hs, is2, is3, p0, p1, p2;
+};
+
+template <typename Real>
+struct Tgt {
+  Real x, y, z, ax, ay, az, hs;
+};
+
+// Accumulator slots.
+enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kW = 12, kE = 15, kNAcc = 18 };
+
+template <unsigned OUT>
+struct Slots {
+  static constexpr bool vel = OUT & VPM_OUT_VEL, str = OUT & VPM_OUT_STRETCH,
+                        pse = OUT & VPM_OUT_PSE, vort = OUT & VPM_OUT_VORT,
+                        ens = OUT & VPM_OUT_ENSTROPHY;
+  static __device__ constexpr bool used(int k) {
+    return (k < 3 && vel) || (k >= 3 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
+           (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
+  }
+};
+
+template <typename Real, unsigned OUT, bool UNIF>
+__device__ __forceinline__ void pair_update(Real* acc, const Tgt<Real>& t,
+                                            const SrcR<Real, UNIF>& s, Real eps_r2) {
+  using S = Slots<OUT>;
+  using M = Arith<Real>;
+  const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+  const Real r2 = dx * dx + dy * dy + dz * dz;
+  Real rho2, inv = Real(1), inv2 = Real(1);
+  if constexpr (UNIF) {
+    rho2 = r2;  // offsets are pre-scaled by 1/sigma
+  } else {
+    const Real sij = t.hs + s.hs;
+    inv = M::rcp(sij);
+    inv2 = inv * inv;
+    rho2 = r2 * inv2;
+  }
+  const Real A = rho2 + Real(1);
+  const Real t1 = M::rsq(A);
+  const Real t2 = t1 * t1;
+  const Real t3 = t2 * t1;
+  const Real t5 = t3 * t2;
+  const Real t7 = t5 * t2;
+  if constexpr (S::vel || S::str) {
+    Real g = (rho2 + Real(2.5)) * t5;
+    if constexpr (!UNIF) g *= inv2 * inv;
+    const Real cx = dy * s.az - dz * s.ay;
+    const Real cy = dz * s.ax - dx * s.az;
+    const Real cz = dx * s.ay - dy * s.ax;
+    if constexpr (S::vel) {
+      acc[kU + 0] += g * cx;
+      acc[kU + 1] += g * cy;
+      acc[kU + 2] += g * cz;
+    }
+    if constexpr (S::str) {
+      acc[kG + 0] += g * s.ax;
+      acc[kG + 1] += g * s.ay;
+      acc[kG + 2] += g * s.az;
+      const Real e = t.ax * cx + t.ay * cy + t.az * cz;
+      Real f2 = (Real(3) * rho2 + Real(10.5)) * t7;
+      if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
+      const Real w = f2 * e;
+      acc[kSd + 0] += w * dx;
+      acc[kSd + 1] += w * dy;
+      acc[kSd + 2] += w * dz;
+    }
+  }
+  if constexpr (S::pse) {
+    Real t9 = t7 * t2;
+    if constexpr (!UNIF) t9 *= inv2;
+    acc[kP + 0] += t9 * (s.ax - t.ax);
+    acc[kP + 1] += t9 * (s.ay - t.ay);
+    acc[kP + 2] += t9 * (s.az - t.az);
+  }
+  if constexpr (S::vort) {
+    Real wv;
+    if constexpr (UNIF) {
+      wv = t7;
+    } else {  // sigma_j, not sigma_ij (ParticlesPhysicsCuda.py:323,334-337)
+      const Real Aj = r2 * s.is2 + Real(1);
+      const Real tj = M::rsq(Aj);
+      const Real tj2 = tj * tj;
+      wv = tj2 * tj2 * tj2 * tj * s.is3;
+    }
+    wv = r2 > eps_r2 ? wv : Real(0);
+    acc[kW + 0] += wv * s.ax;
+    acc[kW + 1] += wv * s.ay;
+    acc[kW + 2] += wv * s.az;
+  }
+  if constexpr (S::ens) {
+    Real en = (rho2 + Real(3.2)) * t3;
+    if constexpr (!UNIF) en *= inv;
+    acc[kE + 0] += en * s.ax;
+    acc[kE + 1] += en * s.ay;
+    acc[kE + 2] += en * s.az;
+  }
+}
+
+template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
+__global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
+  static_assert(TILE % SUB == 0 && TILE % BLOCK == 0, "tile shape");
+  using S = Slots<OUT>;
+  constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
+  constexpr bool kFold32 = std::is_same<Real, float>::value;
+  __shared__ SrcR<Real, UNIF> lds[TILE];
+  __shared__ double red[6][BLOCK / 64];
+  __shared__ double centre[3];
+
+  const int tid = threadIdx.x;
+  const int base = blockIdx.x * (BLOCK * R);
+
+  // ---- targets and the block centre ------------------------------------
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  Src8 tg[R];
+  bool valid[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k = base + r * BLOCK + tid;
+    valid[r] = k < a.n;
+    if (valid[r]) {
+      tg[r] = a.src[k];
+      mn[0] = fmin(mn[0], tg[r].x); mx[0] = fmax(mx[0], tg[r].x);
+      mn[1] = fmin(mn[1], tg[r].y); mx[1] = fmax(mx[1], tg[r].y);
+      mn[2] = fmin(mn[2], tg[r].z); mx[2] = fmax(mx[2], tg[r].z);
+    } else {
+      tg[r] = Src8{0, 0, 0, 0, 0, 0, 1, 0};
+    }
+  }
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double lo = wave_min(mn[c]), hi = wave_max(mx[c]);
+      if (lane == 0) {
+        red[c][wave] = lo;
+        red[3 + c][wave] = hi;
+      }
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double lo = red[tid][0], hi = red[3 + tid][0];
+      for (int w = 1; w < BLOCK / 64; ++w) {
+        lo = fmin(lo, red[tid][w]);
+        hi = fmax(hi, red[3 + tid][w]);
+      }
+      centre[tid] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+  }
+  const double cx = centre[0], cy = centre[1], cz = centre[2];
+  // Uniform sigma: every record holds the same radius; offsets are scaled by
+  // 1/sigma so that r^2 below is rho^2.  The vorticity mask r^2 > 1e-8
+  // (ParticlesPhysicsCuda.py:332) is rescaled alike.
+  const double sigma0 = UNIF ? a.src[0].s : 1.0;
+  const double scale = UNIF ? 1.0 / sigma0 : 1.0;
+  const double eps_r2_d = kEpsPair * scale * scale;
+
+  Tgt<Real> t[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    t[r].x = static_cast<Real>((tg[r].x - cx) * scale);
+    t[r].y = static_cast<Real>((tg[r].y - cy) * scale);
+    t[r].z = static_cast<Real>((tg[r].z - cz) * scale);
+    t[r].ax = static_cast<Real>(tg[r].ax);
+    t[r].ay = static_cast<Real>(tg[r].ay);
+    t[r].az = static_cast<Real>(tg[r].az);
+    t[r].hs = static_cast<Real>(0.5 * tg[r].s);
+  }
+  const Real eps_r2 = static_cast<Real>(eps_r2_d);
+
+  Real part[R][kNAcc];
+  Real ksum[R][kNAcc];
+  Real kcmp[R][kNAcc];
+  double tot[R][kNAcc];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int q = 0; q < kNAcc; ++q) {
+      part[r][q] = Real(0);
+      ksum[r][q] = Real(0);
+      kcmp[r][q] = Real(0);
+      tot[r][q] = 0.0;
+    }
+
+  // ---- sweep the source tiles ------------------------------------------
+  for (int t0 = 0; t0 < a.n; t0 += TILE) {
+    __syncthreads();
+#pragma unroll
+    for (int k = tid; k < TILE; k += BLOCK) {
+      const int j = t0 + k;
+      SrcR<Real, UNIF> s;
+      if (j < a.n) {
+        const Src8 g = a.src[j];
+        s.x = static_cast<Real>((g.x - cx) * scale);
+        s.y = static_cast<Real>((g.y - cy) * scale);
+        s.z = static_cast<Real>((g.z - cz) * scale);
+        s.ax = static_cast<Real>(g.ax);
+        s.ay = static_cast<Real>(g.ay);
+        s.az = static_cast<Real>(g.az);
+        if constexpr (!UNIF) {
+          s.hs = static_cast<Real>(0.5 * g.s);
+          s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
+          s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
+        }
+      } else {  // padding: far away, zero strength -> contributes exactly 0
+        s.x = s.y = s.z = static_cast<Real>(1e18);
+        s.ax = s.ay = s.az = Real(0);
+        if constexpr (!UNIF) {
+          s.hs = Real(0.5);
+          s.is2 = Real(1);
+          s.is3 = Real(1);
+        }
+      }
+      lds[k] = s;
+    }
+    __syncthreads();
+
+    for (int sb = 0; sb < TILE; sb += SUB) {
+#pragma unroll 8
+      for (int q = 0; q < SUB; ++q) {
+        const SrcR<Real, UNIF> s = lds[sb + q];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair_update<Real, OUT, UNIF>(part[r], t[r], s, eps_r2);
+      }
+      if constexpr (kFold32) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int q = 0; q < kNAcc; ++q) {
+            if (!S::used(q)) continue;
+            if constexpr (kComp) {  // Kahan: ksum + kcmp carries the tile sum
+              const Real y = part[r][q] - kcmp[r][q];
+              const Real z = ksum[r][q] + y;
+              kcmp[r][q] = (z - ksum[r][q]) - y;
+              ksum[r][q] = z;
+            } else {
+              tot[r][q] += static_cast<double>(part[r][q]);
+            }
+            part[r][q] = Real(0);
+          }
+      }
+    }
+    if constexpr (kComp) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < kNAcc; ++q) {
+          if (!S::used(q)) continue;
+          tot[r][q] += static_cast<double>(ksum[r][q]) - static_cast<double>(kcmp[r][q]);
+          ksum[r][q] = Real(0);
+          kcmp[r][q] = Real(0);
+        }
+    }
+  }
+  if constexpr (!kFold32) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < kNAcc; ++q) tot[r][q] = static_cast<double>(part[r][q]);
+  }
+
+  // ---- epilogue ----------------------------------------------------------
+  const double C = kOneOverFourPi;
+  double s1 = 1.0, s2 = 1.0, s3 = 1.0;  // 1/sigma^k for uniform sigma
+  if constexpr (UNIF) {
+    s1 = scale;
+    s2 = scale * scale;
+    s3 = s2 * scale;
+  }
+  double ens_local = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!valid[r]) continue;
+    const int k = base + r * BLOCK + tid;
+    const int64_t o = a.perm ? a.perm[k] : k;
+    const double* T = tot[r];
+    if constexpr (S::vel) {
+      const double f = -C * s2;
+      a.vel[o] = f * T[kU + 0];
+      a.vel[a.ld_vel + o] = f * T[kU + 1];
+      a.vel[2 * a.ld_vel + o] = f * T[kU + 2];
+    }
+    if constexpr (S::str) {
+      const double ax = tg[r].ax, ay = tg[r].ay, az = tg[r].az;
+      const double gx = T[kG + 0], gy = T[kG + 1], gz = T[kG + 2];
+      const double f = C * s3;
+      a.str[o] = f * ((ay * gz - az * gy) + T[kSd + 0]);
+      a.str[a.ld_str + o] = f * ((az * gx - ax * gz) + T[kSd + 1]);
+      a.str[2 * a.ld_str + o] = f * ((ax * gy - ay * gx) + T[kSd + 2]);
+    }
+    if constexpr (S::pse) {
+      const double f = 105.0 * tg[r].nu * C * s2;
+      a.pse[o] = f * T[kP + 0];
+      a.pse[a.ld_pse + o] = f * T[kP + 1];
+      a.pse[2 * a.ld_pse + o] = f * T[kP + 2];
+    }
+    if constexpr (S::vort) {
+      const double f = 7.5 * C * s3;
+      a.vort[o] = f * T[kW + 0];
+      a.vort[a.ld_vort + o] = f * T[kW + 1];
+      a.vort[2 * a.ld_vort + o] = f * T[kW + 2];
+    }
+    if constexpr (S::ens) {
+      ens_local += (tg[r].ax * T[kE + 0] + tg[r].ay * T[kE + 1] + tg[r].az * T[kE + 2]) * s1;
+    }
+  }
+  if constexpr (S::ens) {
+    __syncthreads();
+    const double bs = block_sum<BLOCK>(ens_local, &red[0][0]);
+    if (tid == 0) a.ens_part[blockIdx.x] = bs;
+  }
+}
+
+// ------------------------------------------------------------------ configs
+struct TileCfg {
+  int block, r, tile, sub, comp;
+};
+// Index 0 is the FP32 default.  Configs 1.. exist for the hot output sets
+// (VEL|STRETCH, VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
+constexpr TileCfg kCfgs[] = {
+    {256, 2, 256, 32, 0}, {256, 4, 256, 32, 0}, {256, 2, 256, 32, 1}, {256, 4, 256, 32, 1},
+    {128, 4, 256, 32, 0}, {256, 1, 256, 64, 0}, {512, 2, 512, 32, 0}, {256, 2, 512, 64, 0},
+};
+constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
+
+template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
+int launch_cfg(const FusedArgs& a, hipStream_t st, int* nblocks) {
+  const int blocks = static_cast<int>(ceil_div(a.n, BLOCK * R));
+  *nblocks = blocks;
+  if (blocks == 0) return VPM_OK;
+  hipLaunchKernelGGL((fused_kernel<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>), dim3(blocks),
+                     dim3(BLOCK), 0, st, a);
+  VPM_LAUNCH_CHECK("fused_kernel");
+  return VPM_OK;
+}
+
+template <unsigned OUT, bool UNIF>
+int launch_f32(const FusedArgs& a, int cfg, hipStream_t st, int* nb) {
+  switch (cfg) {
+#define VPM_CFG(I)                                                                          \
+  case I:                                                                                    \
+    return launch_cfg<float, OUT, UNIF, kCfgs[I].block, kCfgs[I].r, kCfgs[I].tile,          \
+                      kCfgs[I].sub, kCfgs[I].comp>(a, st, nb);
+    VPM_CFG(0)
+    default:
+      break;
+  }
+  if constexpr (UNIF && (OUT == 3u || OUT == 7u || OUT == 27u)) {
+    switch (cfg) {
+      VPM_CFG(1) VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7)
+      default:
+        break;
+    }
+#undef VPM_CFG
+  }
+  return launch_cfg<float, OUT, UNIF, kCfgs[0].block, kCfgs[0].r, kCfgs[0].tile, kCfgs[0].sub,
+                    kCfgs[0].comp>(a, st, nb);
+}
+
+template <unsigned OUT>
+int launch_mask(const FusedArgs& a, int prec, bool unif, int cfg, hipStream_t st, int* nb) {
+  if (prec == VPM_PREC_F32C) return unif ? launch_f32<OUT, true>(a, cfg, st, nb)
+                                         : launch_f32<OUT, false>(a, cfg, st, nb);
+  if (unif)
+    return launch_cfg<double, OUT, true, kCfgF64.block, kCfgF64.r, kCfgF64.tile, kCfgF64.sub,
+                      kCfgF64.comp>(a, st, nb);
+  return launch_cfg<double, OUT, false, kCfgF64.block, kCfgF64.r, kCfgF64.tile, kCfgF64.sub,
+                    kCfgF64.comp>(a, st, nb);
+}
+
+// Output sets with their own instantiation; any other non-empty set runs the
+// smallest superset and the extra outputs are not stored.
+constexpr unsigned kMasks[] = {1u, 2u, 3u, 4u, 7u, 8u, 16u, 24u, 27u, 31u};
+
+unsigned kernel_mask(unsigned want) {
+  for (unsigned m : kMasks)
+    if ((m & want) == want) return m;
+  return 31u;
+}
+
+int dispatch(const FusedArgs& a, unsigned mask, int prec, bool unif, int cfg, hipStream_t st,
+             int* nb) {
+  switch (mask) {
+    case 1u: return launch_mask<1u>(a, prec, unif, cfg, st, nb);
+    case 2u: return launch_mask<2u>(a, prec, unif, cfg, st, nb);
+    case 3u: return launch_mask<3u>(a, prec, unif, cfg, st, nb);
+    case 4u: return launch_mask<4u>(a, prec, unif, cfg, st, nb);
+    case 7u: return launch_mask<7u>(a, prec, unif, cfg, st, nb);
+    case 8u: return launch_mask<8u>(a, prec, unif, cfg, st, nb);
+    case 16u: return launch_mask<16u>(a, prec, unif, cfg, st, nb);
+    case 24u: return launch_mask<24u>(a, prec, unif, cfg, st, nb);
+    case 27u: return launch_mask<27u>(a, prec, unif, cfg, st, nb);
+    default: return launch_mask<31u>(a, prec, unif, cfg, st, nb);
+  }
+}
+
+// ------------------------------------------------------------------ workspace
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct WsLayout {
+  size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, scratch_vec, cub,
+      total;
+  size_t cub_bytes;
+};
+
+int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L) {
+  const size_t N = static_cast<size_t>(n > 0 ? n : 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align256(bytes);
+    return at;
+  };
+  L->bbox_part = take(kBboxBlocks * 6 * sizeof(double));
+  L->bbox = take(8 * sizeof(double));
+  L->keys_in = take(N * sizeof(uint32_t));
+  L->keys_out = take(N * sizeof(uint32_t));
+  L->idx_in = take(N * sizeof(int));
+  L->perm = take(N * sizeof(int));
+  L->src8 = take(N * sizeof(Src8));
+  L->ens_part = take((N / 64 + 2) * sizeof(double));
+  // Unrequested outputs of a superset kernel land here: 3 vectors of N.
+  L->scratch_vec = take(3 * N * sizeof(double));
+  size_t cub_bytes = 0;
+  if (!o || !o->presorted) {
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, cub_bytes, static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+        static_cast<const int*>(nullptr), static_cast<int*>(nullptr), static_cast<int>(N), 0, 30,
+        hipStream_t(0));
+    if (e != hipSuccess) {
+      set_error("hipcub SortPairs size query failed: %s", hipGetErrorString(e));
+      return VPM_ERR_LAUNCH;
+    }
+  }
+  L->cub_bytes = cub_bytes;
+  L->cub = take(cub_bytes + 16);
+  L->total = off;
+  return VPM_OK;
+}
+
+}  // namespace
+
+}  // namespace vpm
+
+using namespace vpm;
+
+extern "C" {
+
+int vpm_tile_config_count(void) { return kNumCfgs; }
+
+int vpm_tile_config_info(int index, int* block, int* r, int* tile, int* sub, int* comp) {
+  clear_error();
+  if (index < 0 || index >= kNumCfgs) return fail(VPM_ERR_ARG, "tile config index out of range");
+  if (block) *block = kCfgs[index].block;
+  if (r) *r = kCfgs[index].r;
+  if (tile) *tile = kCfgs[index].tile;
+  if (sub) *sub = kCfgs[index].sub;
+  if (comp) *comp = kCfgs[index].comp;
+  return VPM_OK;
+}
+
+size_t vpm_forward_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts) {
+  WsLayout L{};
+  if (layout(n, opts, &L) != VPM_OK) return 0;
+  return L.total;
+}
+
+namespace {
+
+// Validation shared by the three forward entry points.
+int check_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, const char* who) {
+  if (!o) {
+    set_error("%s: null opts", who);
+    return VPM_ERR_ARG;
+  }
+  const unsigned want = o->outputs;
+  if (want == 0 || (want & ~31u)) {
+    set_error("%s: bad output mask %u", who, want);
+    return VPM_ERR_ARG;
+  }
+  if (o->precision != VPM_PREC_F64 && o->precision != VPM_PREC_F32C) {
+    set_error("%s: unknown precision", who);
+    return VPM_ERR_ARG;
+  }
+  if (o->sigma_mode != VPM_SIGMA_GENERAL && o->sigma_mode != VPM_SIGMA_UNIFORM) {
+    set_error("%s: unknown sigma mode", who);
+    return VPM_ERR_ARG;
+  }
+  if (o->tile_config < -1 || o->tile_config >= kNumCfgs) {
+    set_error("%s: tile_config out of range", who);
+    return VPM_ERR_ARG;
+  }
+  return check_particles(p, (want & VPM_OUT_PSE) != 0, who);
+}
+
+int check_ws(int64_t n, const vpm_forward_opts_t* o, void* ws, size_t ws_bytes, WsLayout* L,
+             const char* who) {
+  int rc = layout(n, o, L);
+  if (rc) return rc;
+  if (ws_bytes < L->total || !ws) {
+    set_error("%s: workspace %zu bytes < required %zu", who, ws_bytes, L->total);
+    return VPM_ERR_WORKSPACE;
+  }
+  return VPM_OK;
+}
+
+}  // namespace
+
+int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, void* workspace,
+                        size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_forward(p, o, "vpm_forward_prepare");
+  if (rc) return rc;
+  const int64_t n = p->n;
+  WsLayout L{};
+  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_prepare"))) return rc;
+  if (n == 0) return VPM_OK;
+  hipStream_t st = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  const int ni = static_cast<int>(n);
+  int* perm = nullptr;
+  if (!o->presorted) {
+    double* bpart = reinterpret_cast<double*>(ws + L.bbox_part);
+    double* bbox = reinterpret_cast<double*>(ws + L.bbox);
+    const int nbb = static_cast<int>(std::min<int64_t>(kBboxBlocks, ceil_div(n, kPreBlock)));
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(nbb), dim3(kPreBlock), 0, st, p->pos, p->ld, ni,
+                       bpart);
+    VPM_LAUNCH_CHECK("bbox_partial_kernel");
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, st, bpart, nbb, bbox);
+    VPM_LAUNCH_CHECK("bbox_final_kernel");
+    uint32_t* kin = reinterpret_cast<uint32_t*>(ws + L.keys_in);
+    uint32_t* kout = reinterpret_cast<uint32_t*>(ws + L.keys_out);
+    int* iin = reinterpret_cast<int*>(ws + L.idx_in);
+    perm = reinterpret_cast<int*>(ws + L.perm);
+    const int nb = static_cast<int>(ceil_div(n, kPreBlock));
+    hipLaunchKernelGGL(morton_kernel, dim3(nb), dim3(kPreBlock), 0, st, p->pos, p->ld, ni, bbox,
+                       kin, iin);
+    VPM_LAUNCH_CHECK("morton_kernel");
+    size_t cb = L.cub_bytes;
+    VPM_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(ws + L.cub, cb, kin, kout, iin, perm, ni, 0, 30,
+                                                   st));
+  }
+  Src8* src8 = reinterpret_cast<Src8*>(ws + L.src8);
+  hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned>(ceil_div(n, kPreBlock))),
+                     dim3(kPreBlock), 0, st, *p, ni, static_cast<const int*>(perm), src8);
+  VPM_LAUNCH_CHECK("gather_kernel");
+  return VPM_OK;
+}
+
+int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_forward(p, o, "vpm_forward_run");
+  if (rc) return rc;
+  if (!out) return fail(VPM_ERR_ARG, "vpm_forward_run: null out");
+  const unsigned want = o->outputs;
+  const int64_t n = p->n;
+  if (out->ld < n) return fail(VPM_ERR_ARG, "vpm_forward_run: out->ld < n");
+  if ((want & VPM_OUT_VEL) && n && !out->vel) return fail(VPM_ERR_ARG, "vpm_forward_run: vel output missing");
+  if ((want & VPM_OUT_STRETCH) && n && !out->stretch) return fail(VPM_ERR_ARG, "vpm_forward_run: stretch output missing");
+  if ((want & VPM_OUT_PSE) && n && !out->pse) return fail(VPM_ERR_ARG, "vpm_forward_run: pse output missing");
+  if ((want & VPM_OUT_VORT) && n && !out->vort) return fail(VPM_ERR_ARG, "vpm_forward_run: vort output missing");
+  if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) return fail(VPM_ERR_ARG, "vpm_forward_run: enstrophy output missing");
+  WsLayout L{};
+  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_run"))) return rc;
+  hipStream_t st = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  if (n == 0) {
+    if (want & VPM_OUT_ENSTROPHY) {
+      hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, out->enstrophy, int64_t(1), 0.0);
+      VPM_LAUNCH_CHECK("fill_kernel");
+    }
+    return VPM_OK;
+  }
+  // Unrequested outputs of a superset kernel share one scratch region; a
+  // thread writes only its own target's rows there, so they cannot race.
+  const unsigned mask = kernel_mask(want);
+  double* scratch = reinterpret_cast<double*>(ws + L.scratch_vec);
+  FusedArgs a{};
+  a.n = static_cast<int>(n);
+  a.src = reinterpret_cast<const Src8*>(ws + L.src8);
+  a.perm = o->presorted ? nullptr : reinterpret_cast<const int*>(ws + L.perm);
+  auto route = [&](unsigned bit, double* dst, double** ptr, int64_t* ld) {
+    if (want & bit) {
+      *ptr = dst;
+      *ld = out->ld;
+    } else {
+      *ptr = scratch;
+      *ld = n;
+    }
+  };
+  route(VPM_OUT_VEL, out->vel, &a.vel, &a.ld_vel);
+  route(VPM_OUT_STRETCH, out->stretch, &a.str, &a.ld_str);
+  route(VPM_OUT_PSE, out->pse, &a.pse, &a.ld_pse);
+  route(VPM_OUT_VORT, out->vort, &a.vort, &a.ld_vort);
+  a.ens_part = reinterpret_cast<double*>(ws + L.ens_part);
+  int nblocks = 0;
+  const int cfg = o->tile_config < 0 ? 0 : o->tile_config;
+  if ((rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st,
+                     &nblocks)))
+    return rc;
+  if (want & VPM_OUT_ENSTROPHY) {
+    const double k = (1.0 / (8.0 * kPi)) * kOneOverFourPi;
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(1024), 0, st, a.ens_part,
+                       static_cast<int64_t>(nblocks), k, out->enstrophy);
+    VPM_LAUNCH_CHECK("final_sum_kernel");
+  }
+  return VPM_OK;
+}
+
+int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,
+                void* workspace, size_t ws_bytes, void* stream) {
+  int rc = vpm_forward_prepare(p, o, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  return vpm_forward_run(p, o, out, workspace, ws_bytes, stream);
+}
+
+}  // extern "C"

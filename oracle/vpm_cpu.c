@@ -1,0 +1,133 @@
+/*
+ * CPU baseline: FP64 C/OpenMP restatement of the reference's Numba CPU plugin
+ * (OpenONDA/solvers/VPM/ParticlesPhysics.py, @njit(parallel=True,
+ * fastmath=True) with prange over targets).  TEST INFRASTRUCTURE: built by
+ * oracle/build_cpu.py, used only by tests and by bench.py's cpu_baseline leg.
+ *
+ * Built with -O3 -ffast-math -fopenmp to mirror fastmath=True and prange.
+ * A**2.5, A**3.5 and A**4.5 are written as A^k sqrt(A), the form LLVM's
+ * fast-math pow simplification gives Numba's code; it also keeps GCC's
+ * vectorised pow out (GCC 11 -O3 -ffast-math -march=x86-64-v3 computes wrong
+ * values with it).
+ * Arrays are the reference's AoS (N,3) float64 rows.  Each function
+ * evaluates the targets listed in tidx[0..m) against all n sources (the
+ * full evaluation is tidx = 0..n-1), so a strided target sample can be timed.
+ */
+#include <math.h>
+#include <stdint.h>
+
+#define ONE_OVER_FOUR_PI 0.079577471545947667884441881686257181
+#define EPSILON 1e-8
+
+static inline double pow25(double a) { return a * a * sqrt(a); }
+static inline double pow35(double a) { return a * a * a * sqrt(a); }
+static inline double pow45(double a) { return (a * a) * (a * a) * sqrt(a); }
+
+/* ParticlesPhysics.py:81-143: sigma = sigma_i for every j != i.  The j loop
+ * is split around j = i instead of skipping it with a branch. */
+static inline void vel_range(int64_t j0, int64_t j1, const double* x, const double* a,
+                             double tx, double ty, double tz, double sigma, double sigma_sq,
+                             double* ux, double* uy, double* uz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int64_t j = j0; j < j1; ++j) {
+    const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double rho_sq = r2 / sigma_sq;
+    const double q = (rho_sq + 2.5) / (pow25(rho_sq + 1.0) * sigma * sigma_sq);
+    const double ax = a[3 * j], ay = a[3 * j + 1], az = a[3 * j + 2];
+    sx -= q * (ry * az - rz * ay);
+    sy -= q * (rz * ax - rx * az);
+    sz -= q * (rx * ay - ry * ax);
+  }
+  *ux += sx;
+  *uy += sy;
+  *uz += sz;
+}
+
+void vpm_cpu_selfinduced_velocity(int64_t n, const double* x, const double* a, const double* s,
+                                  int64_t m, const int64_t* tidx, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double sigma = s[i];
+    const double sigma_sq = sigma * sigma;
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    vel_range(0, i, x, a, tx, ty, tz, sigma, sigma_sq, &ux, &uy, &uz);
+    vel_range(i + 1, n, x, a, tx, ty, tz, sigma, sigma_sq, &ux, &uy, &uz);
+    out[3 * k] = ux * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = uy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = uz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ParticlesPhysics.py:268-312 (sigma_ij, j != i). */
+void vpm_cpu_strength_gradients(int64_t n, const double* x, const double* a, const double* s,
+                                int64_t m, const int64_t* tidx, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double ix = a[3 * i], iy = a[3 * i + 1], iz = a[3 * i + 2];
+    const double si = s[i];
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double d2 = rx * rx + ry * ry + rz * rz;
+      const double sigma = (si + s[j]) * 0.5;
+      const double sigma_cb = sigma * sigma * sigma;
+      const double A = d2 / (sigma * sigma) + 1.0;
+      const double f1 = (A + 1.5) / (sigma_cb * pow25(A));
+      const double f2 = 3.0 * (A + 2.5) / (sigma_cb * sigma * sigma * pow35(A));
+      const double ax = a[3 * j], ay = a[3 * j + 1], az = a[3 * j + 2];
+      const double p1x = iy * az - iz * ay, p1y = iz * ax - ix * az, p1z = ix * ay - iy * ax;
+      const double p2x = ry * az - rz * ay, p2y = rz * ax - rx * az, p2z = rx * ay - ry * ax;
+      const double p3 = ix * p2x + iy * p2y + iz * p2z;
+      dx += f1 * p1x + f2 * p3 * rx;
+      dy += f1 * p1y + f2 * p3 * ry;
+      dz += f1 * p1z + f2 * p3 * rz;
+    }
+    out[3 * k] = dx * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = dy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = dz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ParticlesPhysics.py:318-382: nu_ij, A + 1e-8, 1/(4pi) at the end. */
+void vpm_cpu_strength_gradients_pse(int64_t n, const double* x, const double* a, const double* s,
+                                    const double* nu, int64_t m, const int64_t* tidx,
+                                    double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double ix = a[3 * i], iy = a[3 * i + 1], iz = a[3 * i + 2];
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double vis = (nu[i] + nu[j]) * 0.5;
+      const double sigma = (s[i] + s[j]) * 0.5;
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double d2 = rx * rx + ry * ry + rz * rz;
+      const double sigma_sq = sigma * sigma, sigma_cb = sigma_sq * sigma;
+      const double A = d2 / sigma_sq + 1.0 + EPSILON;
+      const double f3 = 105.0 * vis / (sigma_sq * sigma_cb * pow45(A));
+      dx += f3 * sigma_cb * (a[3 * j] - ix);
+      dy += f3 * sigma_cb * (a[3 * j + 1] - iy);
+      dz += f3 * sigma_cb * (a[3 * j + 2] - iz);
+    }
+    out[3 * k] = dx * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = dy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = dz * ONE_OVER_FOUR_PI;
+  }
+}
+
+int vpm_cpu_max_threads(void) {
+#ifdef _OPENMP
+  extern int omp_get_max_threads(void);
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,74 @@
+"""CPU checks of the native boundary: the library loads and exports exactly
+what include/vpm_hip.h declares, with the ctypes signatures the binding uses.
+No kernel is launched.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from openonda_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "vpm_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(vpm_[a-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "vpm_forward" in names and "vpm_get_selfinduced_velocity" in names
+    assert len(names) >= 15
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    lib = ctypes.CDLL(str(L.lib_path()))
+    names = declared_functions()
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in vpm_hip.h but not exported"
+        assert n in L.SIGNATURES, f"{n} has no ctypes signature in openonda_amd/_lib.py"
+    for n in L.SIGNATURES:
+        assert n in names, f"{n} bound in _lib.py but not declared in vpm_hip.h"
+
+
+def test_load_and_version():
+    lib = L.load()
+    assert lib.vpm_abi_version() == 1
+    assert lib.vpm_device_count() >= 0
+    assert lib.vpm_last_error() is not None
+
+
+def test_host_side_queries_without_gpu():
+    lib = L.load()
+    assert lib.vpm_reduction_workspace_bytes(1000) >= 8
+    # presorted: no radix-sort size query (that one asks the device)
+    opts = L.ForwardOpts(L.OUT_VEL | L.OUT_STRETCH, L.PREC_F32C, L.SIGMA_UNIFORM, 1, -1)
+    nb = lib.vpm_forward_workspace_bytes(1 << 20, ctypes.byref(opts))
+    # sorted FP64 records (64 B) + keys/indices + scratch: a few hundred bytes per particle
+    assert 64 * (1 << 20) < nb < 1024 * (1 << 20)
+    assert lib.vpm_tile_config_count() >= 1
+
+
+def test_argument_errors_return_status_without_launch():
+    lib = L.load()
+    # null particle set -> VPM_ERR_ARG, message set, nothing launched
+    rc = lib.vpm_get_selfinduced_velocity(None, 0, None, 0, None)
+    assert rc == L.VPM_ERR_ARG
+    assert b"null" in lib.vpm_last_error()
+    p = L.Particles(10, 5, None, None, None, None)  # ld < n
+    rc = lib.vpm_get_strength_gradients(ctypes.byref(p), None, 10, None)
+    assert rc == L.VPM_ERR_ARG
+    assert b"ld" in lib.vpm_last_error()
+    opts = L.ForwardOpts(0, L.PREC_F32C, L.SIGMA_UNIFORM, 0, -1)
+    out = L.ForwardOut(0, None, None, None, None, None)
+    p0 = L.Particles(0, 0, None, None, None, None)
+    rc = lib.vpm_forward(ctypes.byref(p0), ctypes.byref(opts), ctypes.byref(out), None, 0, None)
+    assert rc == L.VPM_ERR_ARG  # empty output mask
+    with pytest.raises(L.VpmError):
+        L.call("vpm_tile_config_info", 1000, None, None, None, None, None)
