@@ -179,6 +179,123 @@ int vpm_tile_config_count(void);
 int vpm_tile_config_info(int index, int* block, int* targets_per_lane, int* tile,
                          int* subtile, int* comp);
 
+/* ---- batched persistent launch ------------------------------------------
+ * No counterpart in the reference (one particle set per call there).  Many
+ * particle sets of uneven size are concatenated into one vpm_particles_t
+ * (set s owns indices [set_off[s], set_off[s+1])), each with a uniform radius
+ * of its own, and evaluated by one persistent launch whose workgroups pull
+ * (set, target block) work items from a device counter.  Work lists are
+ * built by the caller: items of one set are consecutive, sets ordered by
+ * decreasing size, item_block[k] counts target blocks of
+ * block*targets_per_lane particles of the tile config used.  Device arrays: */
+typedef struct vpm_batch_desc {
+  int32_t nsets;
+  int32_t n_items;
+  int32_t grid;                   /* persistent workgroups; 0 = occupancy x CUs */
+  const int32_t* set_off;         /* [nsets+1] */
+  const int32_t* part_set;        /* [n] set of each particle */
+  const int32_t* item_set;        /* [n_items] */
+  const int32_t* item_block;      /* [n_items] */
+  const int32_t* set_item_first;  /* [nsets] */
+  const int32_t* set_item_count;  /* [nsets] */
+} vpm_batch_desc_t;
+
+size_t vpm_batch_workspace_bytes(int64_t n_total, int nsets, int n_items);
+/* resort=1: per-set Morton sort, then pack; resort=0: pack with the
+ * permutation stored in the workspace by the last resort=1 call (no sort
+ * kernels, so the call can be captured in a HIP graph). */
+int vpm_batch_prepare(const vpm_particles_t* p, const vpm_batch_desc_t* d, int resort,
+                      void* workspace, size_t ws_bytes, void* stream);
+/* One persistent launch of the fused pass over every set (opts.sigma_mode
+ * must be UNIFORM: uniform per set).  out->enstrophy, if requested, receives
+ * nsets values. */
+int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d,
+                  const vpm_forward_opts_t* opts, vpm_forward_out_t* out, void* workspace,
+                  size_t ws_bytes, void* stream);
+/* Per-set dynamic LES filter (vpmModule.py:887-962), one workgroup per set:
+ * state[3*s..] = {C_nu, enstrophy_ref, has_ref}; writes nu_t (if the set's
+ * max exceeds 1e-9) and nu_eff = nu + nu_t. */
+int vpm_batch_les_update(const vpm_batch_desc_t* d, int64_t ld, const double* vorticity,
+                         const double* radii, const double* enstrophy, double* state,
+                         const double* nu, double* nu_t, double* nu_eff, void* workspace,
+                         size_t ws_bytes, int64_t n_total, void* stream);
+
+/* ---- adjoint (vector-Jacobian product) ----------------------------------
+ * No counterpart in the reference (which has no gradients).  For
+ * L = sum_k lam_k.u_k + mu_k.S_k + xi_k.D_k over the fused outputs (GPU
+ * arithmetic: sigma_ij, nu_i; sigma held fixed) writes dL/dx and dL/dalpha.
+ * A NULL cotangent means zero.  opts: precision, sigma_mode, presorted are
+ * used as in vpm_forward (outputs and tile_config are ignored). */
+typedef struct vpm_adjoint_in {
+  int64_t ld;           /* component stride of the cotangents, >= n */
+  const double* lam;    /* [3][ld] cotangent of the velocity, or NULL */
+  const double* mu;     /* [3][ld] cotangent of the stretching, or NULL */
+  const double* xi;     /* [3][ld] cotangent of the PSE term, or NULL (needs p->nu) */
+} vpm_adjoint_in_t;
+
+typedef struct vpm_adjoint_out {
+  int64_t ld;           /* component stride, >= n */
+  double* dpos;         /* [3][ld] dL/dx, overwritten */
+  double* dstr;         /* [3][ld] dL/dalpha, overwritten */
+} vpm_adjoint_out_t;
+
+size_t vpm_adjoint_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts);
+int vpm_adjoint(const vpm_particles_t* p, const vpm_adjoint_in_t* in,
+                const vpm_forward_opts_t* opts, vpm_adjoint_out_t* out, void* workspace,
+                size_t ws_bytes, void* stream);
+
+/* Adjoint of vpm_get_induced_velocity_at w.r.t. the sources (targets fixed):
+ * given lam[3][ld_lam] at the m targets, writes dL/dx and dL/dalpha of the
+ * n sources (FP64).  No reference counterpart. */
+int vpm_induced_velocity_at_adjoint(const vpm_points_t* targets, const vpm_particles_t* src,
+                                    const double* lam, int64_t ld_lam, double* dpos, double* dstr,
+                                    int64_t ld_out, void* stream);
+
+/* ---- device-resident time integration (all FP64, no host sync) ---------
+ * The O(N) parts of a reference solver step, so a step can run (and be
+ * captured in a HIP graph) without leaving the device. */
+
+/* out = x + c*y for [3][ld] vectors (RK stage positions, vpmModule.py:604,637,676). */
+int vpm_axpy3(int64_t n, int64_t ld, const double* x, const double* y, double c, double* out,
+              void* stream);
+/* out = x0 + dt/6 (k1 + 2 k2 + 2 k3 + k4), vpmModule.py:688. */
+int vpm_rk4_combine(int64_t n, int64_t ld, const double* x0, const double* k1, const double* k2,
+                    const double* k3, const double* k4, double dt, double* out, void* stream);
+/* Step-indexed strength update (Euler, then the BDF-shaped explicit
+ * combinations of steps 1, 2, >=3) and history push, in place:
+ * vpmModule.py:749-791 (stretching) and :797-845 (PSE). */
+int vpm_strength_update(int64_t n, int64_t ld, int step, double dt, double* strengths,
+                        double* prev1, double* prev2, double* prev3, const double* rhs,
+                        void* stream);
+/* sigma <- sqrt(sigma^2 + 5.34 (nu + nu_t) dt), vpmModule.py:851-862 (nu_t may be NULL). */
+int vpm_core_spreading(int64_t n, double* sigma, const double* nu, const double* nu_t, double dt,
+                       void* stream);
+/* alpha <- 0.95 alpha + 0.05 (omega/|omega|)|alpha|, vpmModule.py:867-883. */
+int vpm_relax_strengths(int64_t n, int64_t ld, double* strengths, const double* vorticity,
+                        void* stream);
+/* Workspace of vpm_diagnostics / vpm_les_update. */
+size_t vpm_state_workspace_bytes(int64_t n);
+/* out10 = [linear impulse (3), angular impulse (3), sum alpha (3), sum |alpha|]
+ * (vpmModule.py:264-284), deterministic. */
+int vpm_diagnostics(int64_t n, int64_t ld, const double* pos, const double* str, double* out10,
+                    void* workspace, size_t ws_bytes, void* stream);
+/* Dynamic LES filter, vpmModule.py:887-962: if update_constant, updates
+ * state3 = {C_nu, enstrophy_ref, has_ref} from the device enstrophy scalar
+ * (:887-909); then writes nu_t (only if its maximum exceeds 1e-9) from the
+ * particle vorticity, radii and state3[0] = C_nu (:916-962). */
+int vpm_les_update(int64_t n, int64_t ld, const double* vorticity, const double* radii,
+                   const double* enstrophy, double* state3, double* nu_t, int update_constant,
+                   void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- HIP graphs ----------------------------------------------------------
+ * Capture every library call issued on `stream` between begin and end into
+ * one executable graph (stream must not be the default stream).  The graph
+ * replays the same kernels on the same buffers. */
+int vpm_graph_begin(void* stream);
+int vpm_graph_end(void* stream, void** graph_exec);
+int vpm_graph_launch(void* graph_exec, void* stream);
+int vpm_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

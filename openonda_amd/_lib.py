@@ -62,6 +62,21 @@ class ForwardOut(C.Structure):
                 ("pse", C.c_void_p), ("vort", C.c_void_p), ("enstrophy", C.c_void_p)]
 
 
+class AdjointIn(C.Structure):
+    _fields_ = [("ld", C.c_int64), ("lam", C.c_void_p), ("mu", C.c_void_p), ("xi", C.c_void_p)]
+
+
+class AdjointOut(C.Structure):
+    _fields_ = [("ld", C.c_int64), ("dpos", C.c_void_p), ("dstr", C.c_void_p)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("nsets", C.c_int32), ("n_items", C.c_int32), ("grid", C.c_int32),
+                ("set_off", C.c_void_p), ("part_set", C.c_void_p), ("item_set", C.c_void_p),
+                ("item_block", C.c_void_p), ("set_item_first", C.c_void_p),
+                ("set_item_count", C.c_void_p)]
+
+
 _P = C.POINTER
 _VP = C.c_void_p
 _I64 = C.c_int64
@@ -85,6 +100,25 @@ SIGNATURES = {
     "vpm_forward": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),
     "vpm_forward_prepare": (C.c_int, [_P(Particles), _P(ForwardOpts), _VP, _SZ, _VP]),
     "vpm_forward_run": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),
+    "vpm_batch_workspace_bytes": (_SZ, [_I64, C.c_int, C.c_int]),
+    "vpm_batch_prepare": (C.c_int, [_P(Particles), _P(BatchDesc), C.c_int, _VP, _SZ, _VP]),
+    "vpm_batch_run": (C.c_int, [_P(Particles), _P(BatchDesc), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),
+    "vpm_batch_les_update": (C.c_int, [_P(BatchDesc), _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _I64, _VP]),
+    "vpm_adjoint_workspace_bytes": (_SZ, [_I64, _P(ForwardOpts)]),
+    "vpm_adjoint": (C.c_int, [_P(Particles), _P(AdjointIn), _P(ForwardOpts), _P(AdjointOut), _VP, _SZ, _VP]),
+    "vpm_induced_velocity_at_adjoint": (C.c_int, [_P(Points), _P(Particles), _VP, _I64, _VP, _VP, _I64, _VP]),
+    "vpm_axpy3": (C.c_int, [_I64, _I64, _VP, _VP, C.c_double, _VP, _VP]),
+    "vpm_rk4_combine": (C.c_int, [_I64, _I64, _VP, _VP, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "vpm_strength_update": (C.c_int, [_I64, _I64, C.c_int, C.c_double, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vpm_core_spreading": (C.c_int, [_I64, _VP, _VP, _VP, C.c_double, _VP]),
+    "vpm_relax_strengths": (C.c_int, [_I64, _I64, _VP, _VP, _VP]),
+    "vpm_state_workspace_bytes": (_SZ, [_I64]),
+    "vpm_diagnostics": (C.c_int, [_I64, _I64, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "vpm_les_update": (C.c_int, [_I64, _I64, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP, _SZ, _VP]),
+    "vpm_graph_begin": (C.c_int, [_VP]),
+    "vpm_graph_end": (C.c_int, [_VP, _P(C.c_void_p)]),
+    "vpm_graph_launch": (C.c_int, [_VP, _VP]),
+    "vpm_graph_destroy": (C.c_int, [_VP]),
     "vpm_tile_config_count": (C.c_int, []),
     "vpm_tile_config_info": (C.c_int, [C.c_int] + [_P(C.c_int)] * 5),
 }

@@ -31,112 +31,11 @@
 #include <type_traits>
 
 #include "vpm_common.h"
+#include "vpm_prepass.h"
 
 namespace vpm {
 
 namespace {
-
-struct alignas(16) Src8 {
-  double x, y, z, ax, ay, az, s, nu;
-};
-
-// ------------------------------------------------------------------ prepass
-constexpr int kPreBlock = 256;
-constexpr int kBboxBlocks = 512;
-
-__global__ __launch_bounds__(kPreBlock) void bbox_partial_kernel(const double* __restrict__ pos,
-                                                                 int64_t ld, int n,
-                                                                 double* __restrict__ part) {
-  __shared__ double red[6][kPreBlock / 64];
-  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int i = blockIdx.x * kPreBlock + threadIdx.x; i < n; i += gridDim.x * kPreBlock) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double v = pos[c * ld + i];
-      mn[c] = fmin(mn[c], v);
-      mx[c] = fmax(mx[c], v);
-    }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    mn[c] = wave_min(mn[c]);
-    mx[c] = wave_max(mx[c]);
-    if (lane == 0) {
-      red[c][wave] = mn[c];
-      red[3 + c][wave] = mx[c];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    double v = red[threadIdx.x][0];
-    for (int w = 1; w < kPreBlock / 64; ++w)
-      v = threadIdx.x < 3 ? fmin(v, red[threadIdx.x][w]) : fmax(v, red[threadIdx.x][w]);
-    part[blockIdx.x * 6 + threadIdx.x] = v;
-  }
-}
-
-__global__ __launch_bounds__(64) void bbox_final_kernel(const double* __restrict__ part, int nb,
-                                                        double* __restrict__ bbox) {
-  if (threadIdx.x < 6) {
-    double v = part[threadIdx.x];
-    for (int b = 1; b < nb; ++b)
-      v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x]) : fmax(v, part[b * 6 + threadIdx.x]);
-    bbox[threadIdx.x] = v;
-  }
-}
-
-__device__ inline uint32_t spread10(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-__global__ __launch_bounds__(kPreBlock) void morton_kernel(const double* __restrict__ pos,
-                                                           int64_t ld, int n,
-                                                           const double* __restrict__ bbox,
-                                                           uint32_t* __restrict__ keys,
-                                                           int* __restrict__ idx) {
-  const int i = blockIdx.x * kPreBlock + threadIdx.x;
-  if (i >= n) return;
-  uint32_t q[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double lo = bbox[c], ext = bbox[3 + c] - lo;
-    const double f = ext > 0.0 ? (pos[c * ld + i] - lo) / ext : 0.0;
-    int v = static_cast<int>(f * 1023.0);
-    v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
-    q[c] = static_cast<uint32_t>(v);
-  }
-  keys[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
-  idx[i] = i;
-}
-
-__global__ __launch_bounds__(kPreBlock) void gather_kernel(const vpm_particles_t p, int n,
-                                                           const int* __restrict__ perm,
-                                                           Src8* __restrict__ out) {
-  const int k = blockIdx.x * kPreBlock + threadIdx.x;
-  if (k >= n) return;
-  const int i = perm ? perm[k] : k;
-  Src8 r;
-  r.x = p.pos[i];
-  r.y = p.pos[p.ld + i];
-  r.z = p.pos[2 * p.ld + i];
-  r.ax = p.str[i];
-  r.ay = p.str[p.ld + i];
-  r.az = p.str[2 * p.ld + i];
-  r.s = p.sigma[i];
-  r.nu = p.nu ? p.nu[i] : 0.0;
-  out[k] = r;
-}
-
-__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 // ------------------------------------------------------------------ main pass
 struct FusedArgs {
@@ -190,15 +89,20 @@ struct Slots {
   static constexpr bool vel = OUT & VPM_OUT_VEL, str = OUT & VPM_OUT_STRETCH,
                         pse = OUT & VPM_OUT_PSE, vort = OUT & VPM_OUT_VORT,
                         ens = OUT & VPM_OUT_ENSTROPHY;
+  // FP32-accumulated slots.  kG (the alpha_j sum of the stretching) is
+  // accumulated in FP64 per pair instead: on symmetric vortex structures the
+  // term alpha_i x G is a near-exact cancellation and FP32 partials of G
+  // cost an order of magnitude in the stretching's relative error.
   static __device__ constexpr bool used(int k) {
-    return (k < 3 && vel) || (k >= 3 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
+    return (k < 3 && vel) || (k >= 6 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
            (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
   }
 };
 
 template <typename Real, unsigned OUT, bool UNIF>
-__device__ __forceinline__ void pair_update(Real* acc, const Tgt<Real>& t,
-                                            const SrcR<Real, UNIF>& s, Real eps_r2) {
+__device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
+                                            const SrcR<Real, UNIF>& s, const double* aj64,
+                                            Real eps_r2) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
@@ -230,9 +134,10 @@ __device__ __forceinline__ void pair_update(Real* acc, const Tgt<Real>& t,
       acc[kU + 2] += g * cz;
     }
     if constexpr (S::str) {
-      acc[kG + 0] += g * s.ax;
-      acc[kG + 1] += g * s.ay;
-      acc[kG + 2] += g * s.az;
+      const double gd = static_cast<double>(g);
+      g64[0] = fma(gd, aj64[0], g64[0]);
+      g64[1] = fma(gd, aj64[1], g64[1]);
+      g64[2] = fma(gd, aj64[2], g64[2]);
       const Real e = t.ax * cx + t.ay * cy + t.az * cz;
       Real f2 = (Real(3) * rho2 + Real(10.5)) * t7;
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
@@ -273,18 +178,22 @@ __device__ __forceinline__ void pair_update(Real* acc, const Tgt<Real>& t,
   }
 }
 
+// One workgroup's work: BLOCK*R consecutive sorted targets of a particle set
+// of n particles (src, perm) starting at `base`, against all n sources.
+// Writes the outputs of its targets and its enstrophy partial to *ens_slot.
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-__global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
+__device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, const Src8* src,
+                                            const int* perm, const int base, double* ens_slot) {
   static_assert(TILE % SUB == 0 && TILE % BLOCK == 0, "tile shape");
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
   constexpr bool kFold32 = std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
+  __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ double red[6][BLOCK / 64];
   __shared__ double centre[3];
 
   const int tid = threadIdx.x;
-  const int base = blockIdx.x * (BLOCK * R);
 
   // ---- targets and the block centre ------------------------------------
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -293,9 +202,9 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = base + r * BLOCK + tid;
-    valid[r] = k < a.n;
+    valid[r] = k < n;
     if (valid[r]) {
-      tg[r] = a.src[k];
+      tg[r] = src[k];
       mn[0] = fmin(mn[0], tg[r].x); mx[0] = fmax(mx[0], tg[r].x);
       mn[1] = fmin(mn[1], tg[r].y); mx[1] = fmax(mx[1], tg[r].y);
       mn[2] = fmin(mn[2], tg[r].z); mx[2] = fmax(mx[2], tg[r].z);
@@ -328,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
   // Uniform sigma: every record holds the same radius; offsets are scaled by
   // 1/sigma so that r^2 below is rho^2.  The vorticity mask r^2 > 1e-8
   // (ParticlesPhysicsCuda.py:332) is rescaled alike.
-  const double sigma0 = UNIF ? a.src[0].s : 1.0;
+  const double sigma0 = UNIF ? src[0].s : 1.0;
   const double scale = UNIF ? 1.0 / sigma0 : 1.0;
   const double eps_r2_d = kEpsPair * scale * scale;
 
@@ -349,6 +258,7 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
   Real ksum[R][kNAcc];
   Real kcmp[R][kNAcc];
   double tot[R][kNAcc];
+  double gsum[R][3];
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -358,22 +268,29 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
       kcmp[r][q] = Real(0);
       tot[r][q] = 0.0;
     }
+#pragma unroll
+  for (int r = 0; r < R; ++r) gsum[r][0] = gsum[r][1] = gsum[r][2] = 0.0;
 
   // ---- sweep the source tiles ------------------------------------------
-  for (int t0 = 0; t0 < a.n; t0 += TILE) {
+  for (int t0 = 0; t0 < n; t0 += TILE) {
     __syncthreads();
 #pragma unroll
     for (int k = tid; k < TILE; k += BLOCK) {
       const int j = t0 + k;
       SrcR<Real, UNIF> s;
-      if (j < a.n) {
-        const Src8 g = a.src[j];
+      if (j < n) {
+        const Src8 g = src[j];
         s.x = static_cast<Real>((g.x - cx) * scale);
         s.y = static_cast<Real>((g.y - cy) * scale);
         s.z = static_cast<Real>((g.z - cz) * scale);
         s.ax = static_cast<Real>(g.ax);
         s.ay = static_cast<Real>(g.ay);
         s.az = static_cast<Real>(g.az);
+        if constexpr (S::str) {
+          lds_a64[k][0] = g.ax;
+          lds_a64[k][1] = g.ay;
+          lds_a64[k][2] = g.az;
+        }
         if constexpr (!UNIF) {
           s.hs = static_cast<Real>(0.5 * g.s);
           s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
@@ -382,6 +299,11 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
       } else {  // padding: far away, zero strength -> contributes exactly 0
         s.x = s.y = s.z = static_cast<Real>(1e18);
         s.ax = s.ay = s.az = Real(0);
+        if constexpr (S::str) {
+          lds_a64[k][0] = 0.0;
+          lds_a64[k][1] = 0.0;
+          lds_a64[k][2] = 0.0;
+        }
         if constexpr (!UNIF) {
           s.hs = Real(0.5);
           s.is2 = Real(1);
@@ -396,8 +318,15 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
 #pragma unroll 8
       for (int q = 0; q < SUB; ++q) {
         const SrcR<Real, UNIF> s = lds[sb + q];
+        double aj64[3] = {0.0, 0.0, 0.0};
+        if constexpr (S::str) {
+          aj64[0] = lds_a64[sb + q][0];
+          aj64[1] = lds_a64[sb + q][1];
+          aj64[2] = lds_a64[sb + q][2];
+        }
 #pragma unroll
-        for (int r = 0; r < R; ++r) pair_update<Real, OUT, UNIF>(part[r], t[r], s, eps_r2);
+        for (int r = 0; r < R; ++r)
+          pair_update<Real, OUT, UNIF>(part[r], gsum[r], t[r], s, aj64, eps_r2);
       }
       if constexpr (kFold32) {
 #pragma unroll
@@ -449,8 +378,9 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
   for (int r = 0; r < R; ++r) {
     if (!valid[r]) continue;
     const int k = base + r * BLOCK + tid;
-    const int64_t o = a.perm ? a.perm[k] : k;
+    const int64_t o = perm ? perm[k] : k;
     const double* T = tot[r];
+    const Src8 ti = src[k];
     if constexpr (S::vel) {
       const double f = -C * s2;
       a.vel[o] = f * T[kU + 0];
@@ -458,15 +388,15 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
       a.vel[2 * a.ld_vel + o] = f * T[kU + 2];
     }
     if constexpr (S::str) {
-      const double ax = tg[r].ax, ay = tg[r].ay, az = tg[r].az;
-      const double gx = T[kG + 0], gy = T[kG + 1], gz = T[kG + 2];
+      const double ax = ti.ax, ay = ti.ay, az = ti.az;
+      const double gx = gsum[r][0], gy = gsum[r][1], gz = gsum[r][2];
       const double f = C * s3;
       a.str[o] = f * ((ay * gz - az * gy) + T[kSd + 0]);
       a.str[a.ld_str + o] = f * ((az * gx - ax * gz) + T[kSd + 1]);
       a.str[2 * a.ld_str + o] = f * ((ax * gy - ay * gx) + T[kSd + 2]);
     }
     if constexpr (S::pse) {
-      const double f = 105.0 * tg[r].nu * C * s2;
+      const double f = 105.0 * ti.nu * C * s2;
       a.pse[o] = f * T[kP + 0];
       a.pse[a.ld_pse + o] = f * T[kP + 1];
       a.pse[2 * a.ld_pse + o] = f * T[kP + 2];
@@ -478,14 +408,20 @@ __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
       a.vort[2 * a.ld_vort + o] = f * T[kW + 2];
     }
     if constexpr (S::ens) {
-      ens_local += (tg[r].ax * T[kE + 0] + tg[r].ay * T[kE + 1] + tg[r].az * T[kE + 2]) * s1;
+      ens_local += (ti.ax * T[kE + 0] + ti.ay * T[kE + 1] + ti.az * T[kE + 2]) * s1;
     }
   }
   if constexpr (S::ens) {
     __syncthreads();
     const double bs = block_sum<BLOCK>(ens_local, &red[0][0]);
-    if (tid == 0) a.ens_part[blockIdx.x] = bs;
+    if (tid == 0) *ens_slot = bs;
   }
+}
+
+template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
+__global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
+  fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
+      a, a.n, a.src, a.perm, static_cast<int>(blockIdx.x) * (BLOCK * R), a.ens_part + blockIdx.x);
 }
 
 // ------------------------------------------------------------------ configs
@@ -573,8 +509,6 @@ int dispatch(const FusedArgs& a, unsigned mask, int prec, bool unif, int cfg, hi
 }
 
 // ------------------------------------------------------------------ workspace
-inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
 struct WsLayout {
   size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, scratch_vec, cub,
       total;
@@ -601,19 +535,290 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L) {
   L->scratch_vec = take(3 * N * sizeof(double));
   size_t cub_bytes = 0;
   if (!o || !o->presorted) {
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
-        nullptr, cub_bytes, static_cast<const uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-        static_cast<const int*>(nullptr), static_cast<int*>(nullptr), static_cast<int>(N), 0, 30,
-        hipStream_t(0));
-    if (e != hipSuccess) {
-      set_error("hipcub SortPairs size query failed: %s", hipGetErrorString(e));
-      return VPM_ERR_LAUNCH;
-    }
+    const int rc = sort_temp_bytes(n, &cub_bytes);
+    if (rc) return rc;
   }
   L->cub_bytes = cub_bytes;
   L->cub = take(cub_bytes + 16);
   L->total = off;
   return VPM_OK;
+}
+
+// ------------------------------------------------------------------ batched
+// Many particle sets ("instances") of uneven size in one persistent launch.
+// The sets are concatenated (set s owns caller indices [off_s, off_s + n_s))
+// and sorted per set (set id in the high bits of a 64-bit Morton key), so a
+// set stays contiguous after sorting.  Work items are (set, target block);
+// the host lists them largest set first and workgroups pull them from a
+// counter, so the long items start first and the tail is short.
+struct BatchArgs {
+  FusedArgs f;            // outputs in caller order; f.ens_part: one slot per item
+  const Src8* src;        // sorted, concatenated
+  const int* perm;        // sorted position -> caller index
+  const int* inst_off;    // [S+1]
+  const int* item_inst;   // [items]
+  const int* item_block;  // [items]
+  int n_items;
+  int* counter;
+};
+
+template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
+__global__ __launch_bounds__(BLOCK) void fused_batch_kernel(BatchArgs b) {
+  __shared__ int item_s;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      item_s = __hip_atomic_fetch_add(b.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int item = item_s;
+    if (item >= b.n_items) break;  // every workgroup ends once the list is drained
+    const int inst = b.item_inst[item];
+    const int off = b.inst_off[inst];
+    const int n = b.inst_off[inst + 1] - off;
+    fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
+        b.f, n, b.src + off, b.perm + off, b.item_block[item] * (BLOCK * R), b.f.ens_part + item);
+  }
+}
+
+__global__ void counter_reset_kernel(int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
+}
+
+// per-set bounding boxes: one block per set
+__global__ __launch_bounds__(256) void set_bbox_kernel(const double* __restrict__ pos, int64_t ld,
+                                                       const int* __restrict__ off,
+                                                       double* __restrict__ bbox) {
+  __shared__ double red[6][4];
+  const int s = blockIdx.x;
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += 256)
+    for (int c = 0; c < 3; ++c) {
+      const double v = pos[c * ld + i];
+      mn[c] = fmin(mn[c], v);
+      mx[c] = fmax(mx[c], v);
+    }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = 0; c < 3; ++c) {
+    const double lo = wave_min(mn[c]), hi = wave_max(mx[c]);
+    if (lane == 0) {
+      red[c][wave] = lo;
+      red[3 + c][wave] = hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = red[threadIdx.x][0];
+    for (int w = 1; w < 4; ++w)
+      v = threadIdx.x < 3 ? fmin(v, red[threadIdx.x][w]) : fmax(v, red[threadIdx.x][w]);
+    bbox[s * 6 + threadIdx.x] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void set_keys_kernel(const double* __restrict__ pos, int64_t ld,
+                                                       int n, const int* __restrict__ part_inst,
+                                                       const double* __restrict__ bbox,
+                                                       uint64_t* __restrict__ keys,
+                                                       int* __restrict__ idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int s = part_inst[i];
+  uint32_t q[3];
+  for (int c = 0; c < 3; ++c) {
+    const double lo = bbox[s * 6 + c], ext = bbox[s * 6 + 3 + c] - lo;
+    const double f = ext > 0.0 ? (pos[c * ld + i] - lo) / ext : 0.0;
+    int v = static_cast<int>(f * 1023.0);
+    v = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+    q[c] = static_cast<uint32_t>(v);
+  }
+  const uint32_t m = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+  keys[i] = (static_cast<uint64_t>(s) << 30) | m;
+  idx[i] = i;
+}
+
+// per-set enstrophy: one block per set sums the partials of its items
+__global__ __launch_bounds__(256) void set_enstrophy_kernel(const double* __restrict__ part,
+                                                            const int* __restrict__ first,
+                                                            const int* __restrict__ count,
+                                                            double scale,
+                                                            double* __restrict__ out) {
+  __shared__ double red[4];
+  const int s = blockIdx.x;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count[s]; k += 256) v += part[first[s] + k];
+  v = block_sum<256>(v, red);
+  if (threadIdx.x == 0) out[s] = v * scale;
+}
+
+// Dynamic LES filter per set, vpmModule.py:887-962, one block per set:
+// C_nu recursion from the set's enstrophy, mean vorticity, nu_t candidates,
+// the max > 1e-9 test and the commit; also writes nu_eff = nu + nu_t.
+__global__ __launch_bounds__(256) void set_les_kernel(const int* __restrict__ off, int64_t ld,
+                                                      const double* __restrict__ w,
+                                                      const double* __restrict__ radii,
+                                                      const double* __restrict__ ens,
+                                                      double* __restrict__ state,
+                                                      const double* __restrict__ nu,
+                                                      double* __restrict__ nu_t,
+                                                      double* __restrict__ nu_eff,
+                                                      double* __restrict__ cand) {
+  __shared__ double red[4];
+  __shared__ double sh[4];
+  const int s = blockIdx.x;
+  const int b = off[s], e = off[s + 1], n = e - b;
+  double* st = state + 3 * s;
+  if (threadIdx.x == 0) {
+    const double en = sqrt(ens[s]);
+    double ref = st[2] != 0.0 ? st[1] : en;
+    ref = 0.1 * en + (1 - 0.1) * ref;
+    double c = st[0] * en / (ref + 1e-9);
+    c = fmax(1e-9, fmin(c, 0.0761));
+    st[0] = c;
+    st[1] = ref;
+    st[2] = 1.0;
+    sh[3] = c;
+  }
+  double m[3] = {0, 0, 0};
+  for (int i = b + threadIdx.x; i < e; i += 256)
+    for (int c = 0; c < 3; ++c) m[c] += w[c * ld + i];
+  for (int c = 0; c < 3; ++c) {
+    const double v = block_sum<256>(m[c], red);
+    if (threadIdx.x == 0) sh[c] = v / n;
+  }
+  __syncthreads();
+  const double mx = sh[0], my = sh[1], mz = sh[2], cnu = sh[3];
+  const double mm = sqrt(mx * mx + my * my + mz * mz);
+  const double ux = mx / mm, uy = my / mm, uz = mz / mm;
+  const double beta0 = kPi / 4.0;
+  double vmax = 0.0;
+  for (int i = b + threadIdx.x; i < e; i += 256) {
+    const double wx = w[i], wy = w[ld + i], wz = w[2 * ld + i];
+    const double wm = sqrt(wx * wx + wy * wy + wz * wz);
+    const double c = (wx / wm) * ux + (wy / wm) * uy + (wz / wm) * uz;
+    const double beta = acos(fmin(fmax(c, -1.0), 1.0));
+    const double psi = (beta0 <= beta && beta <= kPi - beta0) ? 1.0 : 0.0;
+    const double cr = cnu * radii[i];
+    const double v = psi * (cr * cr) * wm;
+    cand[i] = v;
+    vmax = fmax(vmax, v);
+  }
+  vmax = wave_max(vmax);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+  __syncthreads();
+  const double top = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  for (int i = b + threadIdx.x; i < e; i += 256) {
+    if (top > 1e-9) nu_t[i] = cand[i];
+    nu_eff[i] = nu[i] + nu_t[i];
+  }
+}
+
+template <typename Real, unsigned OUT, int BLOCK, int R, int TILE, int SUB, int COMP>
+int launch_batch_cfg(const BatchArgs& b, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(counter_reset_kernel, dim3(1), dim3(64), 0, st, b.counter);
+  VPM_LAUNCH_CHECK("counter_reset_kernel");
+  if (b.n_items == 0) return VPM_OK;
+  if (grid <= 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    VPM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, fused_batch_kernel<Real, OUT, true, BLOCK, R, TILE, SUB, COMP>, BLOCK, 0));
+    VPM_HIP_TRY(hipGetDevice(&dev));
+    VPM_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid = std::max(1, per_cu) * std::max(1, cus);
+  }
+  grid = std::min(grid, b.n_items);
+  hipLaunchKernelGGL((fused_batch_kernel<Real, OUT, true, BLOCK, R, TILE, SUB, COMP>), dim3(grid),
+                     dim3(BLOCK), 0, st, b);
+  VPM_LAUNCH_CHECK("fused_batch_kernel");
+  return VPM_OK;
+}
+
+template <unsigned OUT>
+int launch_batch_mask(const BatchArgs& b, int prec, int cfg, int grid, hipStream_t st) {
+  if (prec == VPM_PREC_F64)
+    return launch_batch_cfg<double, OUT, kCfgF64.block, kCfgF64.r, kCfgF64.tile, kCfgF64.sub,
+                            kCfgF64.comp>(b, grid, st);
+#define VPM_BCFG(I)                                                                            \
+  case I:                                                                                      \
+    return launch_batch_cfg<float, OUT, kCfgs[I].block, kCfgs[I].r, kCfgs[I].tile,             \
+                            kCfgs[I].sub, kCfgs[I].comp>(b, grid, st);
+  switch (cfg) {
+    VPM_BCFG(1) VPM_BCFG(2) VPM_BCFG(3) VPM_BCFG(4) VPM_BCFG(5) VPM_BCFG(6) VPM_BCFG(7)
+    default:
+      break;
+  }
+#undef VPM_BCFG
+  return launch_batch_cfg<float, OUT, kCfgs[0].block, kCfgs[0].r, kCfgs[0].tile, kCfgs[0].sub,
+                          kCfgs[0].comp>(b, grid, st);
+}
+
+int dispatch_batch(const BatchArgs& b, unsigned mask, int prec, int cfg, int grid,
+                   hipStream_t st) {
+  switch (mask) {
+    case 3u: return launch_batch_mask<3u>(b, prec, cfg, grid, st);
+    case 7u: return launch_batch_mask<7u>(b, prec, cfg, grid, st);
+    case 27u: return launch_batch_mask<27u>(b, prec, cfg, grid, st);
+    default: return launch_batch_mask<31u>(b, prec, cfg, grid, st);
+  }
+}
+
+unsigned batch_mask(unsigned want) {
+  for (unsigned m : {3u, 7u, 27u, 31u})
+    if ((m & want) == want) return m;
+  return 31u;
+}
+
+struct BatchLayout {
+  size_t bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, cand, scratch_vec, counter, cub,
+      total, cub_bytes;
+};
+
+int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L) {
+  const size_t N = static_cast<size_t>(n > 0 ? n : 1);
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t at = off;
+    off += align256(b);
+    return at;
+  };
+  L->bbox = take(static_cast<size_t>(nsets > 0 ? nsets : 1) * 6 * sizeof(double));
+  L->keys_in = take(N * 8);
+  L->keys_out = take(N * 8);
+  L->idx_in = take(N * 4);
+  L->perm = take(N * 4);
+  L->src8 = take(N * sizeof(Src8));
+  L->ens_part = take(static_cast<size_t>(n_items > 0 ? n_items : 1) * sizeof(double));
+  L->cand = take(N * sizeof(double));
+  L->scratch_vec = take(3 * N * sizeof(double));
+  L->counter = take(64);
+  size_t cb = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, cb, static_cast<const uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+      static_cast<const int*>(nullptr), static_cast<int*>(nullptr), static_cast<int>(N), 0, 62,
+      hipStream_t(0));
+  if (e != hipSuccess) {
+    set_error("hipcub SortPairs size query failed: %s", hipGetErrorString(e));
+    return VPM_ERR_LAUNCH;
+  }
+  L->cub_bytes = cb;
+  L->cub = take(cb + 16);
+  L->total = off;
+  return VPM_OK;
+}
+
+int check_desc(const vpm_batch_desc_t* d, const vpm_particles_t* p, const char* who) {
+  if (!d || d->nsets < 0 || d->n_items < 0) {
+    set_error("%s: bad batch descriptor", who);
+    return VPM_ERR_ARG;
+  }
+  if (d->nsets > (1 << 24)) {
+    set_error("%s: too many sets", who);
+    return VPM_ERR_ARG;
+  }
+  if (d->nsets > 0 && (!d->set_off || !d->part_set || !d->item_set || !d->item_block ||
+                       !d->set_item_first || !d->set_item_count)) {
+    set_error("%s: null descriptor array", who);
+    return VPM_ERR_ARG;
+  }
+  return p ? check_particles(p, false, who) : VPM_OK;
 }
 
 }  // namespace
@@ -698,25 +903,12 @@ int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, v
   const int ni = static_cast<int>(n);
   int* perm = nullptr;
   if (!o->presorted) {
-    double* bpart = reinterpret_cast<double*>(ws + L.bbox_part);
-    double* bbox = reinterpret_cast<double*>(ws + L.bbox);
-    const int nbb = static_cast<int>(std::min<int64_t>(kBboxBlocks, ceil_div(n, kPreBlock)));
-    hipLaunchKernelGGL(bbox_partial_kernel, dim3(nbb), dim3(kPreBlock), 0, st, p->pos, p->ld, ni,
-                       bpart);
-    VPM_LAUNCH_CHECK("bbox_partial_kernel");
-    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, st, bpart, nbb, bbox);
-    VPM_LAUNCH_CHECK("bbox_final_kernel");
-    uint32_t* kin = reinterpret_cast<uint32_t*>(ws + L.keys_in);
-    uint32_t* kout = reinterpret_cast<uint32_t*>(ws + L.keys_out);
-    int* iin = reinterpret_cast<int*>(ws + L.idx_in);
     perm = reinterpret_cast<int*>(ws + L.perm);
-    const int nb = static_cast<int>(ceil_div(n, kPreBlock));
-    hipLaunchKernelGGL(morton_kernel, dim3(nb), dim3(kPreBlock), 0, st, p->pos, p->ld, ni, bbox,
-                       kin, iin);
-    VPM_LAUNCH_CHECK("morton_kernel");
-    size_t cb = L.cub_bytes;
-    VPM_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(ws + L.cub, cb, kin, kout, iin, perm, ni, 0, 30,
-                                                   st));
+    rc = morton_sort(p, reinterpret_cast<double*>(ws + L.bbox_part),
+                     reinterpret_cast<double*>(ws + L.bbox), reinterpret_cast<uint32_t*>(ws + L.keys_in),
+                     reinterpret_cast<uint32_t*>(ws + L.keys_out), reinterpret_cast<int*>(ws + L.idx_in),
+                     perm, ws + L.cub, L.cub_bytes, st);
+    if (rc) return rc;
   }
   Src8* src8 = reinterpret_cast<Src8*>(ws + L.src8);
   hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned>(ceil_div(n, kPreBlock))),
@@ -791,6 +983,132 @@ int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forwa
   int rc = vpm_forward_prepare(p, o, workspace, ws_bytes, stream);
   if (rc) return rc;
   return vpm_forward_run(p, o, out, workspace, ws_bytes, stream);
+}
+
+size_t vpm_batch_workspace_bytes(int64_t n_total, int nsets, int n_items) {
+  BatchLayout L{};
+  if (batch_layout(n_total, nsets, n_items, &L) != VPM_OK) return 0;
+  return L.total;
+}
+
+int vpm_batch_prepare(const vpm_particles_t* p, const vpm_batch_desc_t* d, int resort,
+                      void* workspace, size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_desc(d, p, "vpm_batch_prepare");
+  if (rc) return rc;
+  BatchLayout L{};
+  if ((rc = batch_layout(p->n, d->nsets, d->n_items, &L))) return rc;
+  if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_prepare: workspace too small");
+  if (p->n == 0) return VPM_OK;
+  hipStream_t st = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  const int ni = static_cast<int>(p->n);
+  int* perm = reinterpret_cast<int*>(ws + L.perm);
+  if (resort) {
+    double* bbox = reinterpret_cast<double*>(ws + L.bbox);
+    hipLaunchKernelGGL(set_bbox_kernel, dim3(d->nsets), dim3(256), 0, st, p->pos, p->ld,
+                       d->set_off, bbox);
+    VPM_LAUNCH_CHECK("set_bbox_kernel");
+    uint64_t* kin = reinterpret_cast<uint64_t*>(ws + L.keys_in);
+    uint64_t* kout = reinterpret_cast<uint64_t*>(ws + L.keys_out);
+    int* iin = reinterpret_cast<int*>(ws + L.idx_in);
+    hipLaunchKernelGGL(set_keys_kernel, dim3(static_cast<unsigned>(ceil_div(p->n, 256))), dim3(256),
+                       0, st, p->pos, p->ld, ni, d->part_set, bbox, kin, iin);
+    VPM_LAUNCH_CHECK("set_keys_kernel");
+    int bits = 30;
+    while ((int64_t(1) << (bits - 30)) < d->nsets) ++bits;
+    size_t cb = L.cub_bytes;
+    VPM_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(ws + L.cub, cb, kin, kout, iin, perm, ni, 0,
+                                                   bits, st));
+  }
+  hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned>(ceil_div(p->n, kPreBlock))),
+                     dim3(kPreBlock), 0, st, *p, ni, static_cast<const int*>(perm),
+                     reinterpret_cast<Src8*>(ws + L.src8));
+  VPM_LAUNCH_CHECK("gather_kernel");
+  return VPM_OK;
+}
+
+int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm_forward_opts_t* o,
+                  vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_desc(d, p, "vpm_batch_run");
+  if (rc) return rc;
+  if (!o || !out) return fail(VPM_ERR_ARG, "vpm_batch_run: null opts/out");
+  const unsigned want = o->outputs;
+  if (want == 0 || (want & ~31u)) return fail(VPM_ERR_ARG, "vpm_batch_run: bad output mask");
+  if (o->sigma_mode != VPM_SIGMA_UNIFORM)
+    return fail(VPM_ERR_UNSUPPORTED, "vpm_batch_run: each set needs a uniform radius (sigma_mode UNIFORM)");
+  if (o->precision != VPM_PREC_F64 && o->precision != VPM_PREC_F32C)
+    return fail(VPM_ERR_ARG, "vpm_batch_run: unknown precision");
+  if (o->tile_config < -1 || o->tile_config >= kNumCfgs)
+    return fail(VPM_ERR_ARG, "vpm_batch_run: tile_config out of range");
+  if ((want & VPM_OUT_PSE) && p->n && !p->nu) return fail(VPM_ERR_ARG, "vpm_batch_run: PSE needs nu");
+  const int64_t n = p->n;
+  if (out->ld < n) return fail(VPM_ERR_ARG, "vpm_batch_run: out->ld < n");
+  if ((want & VPM_OUT_VEL) && n && !out->vel) return fail(VPM_ERR_ARG, "vpm_batch_run: vel missing");
+  if ((want & VPM_OUT_STRETCH) && n && !out->stretch) return fail(VPM_ERR_ARG, "vpm_batch_run: stretch missing");
+  if ((want & VPM_OUT_PSE) && n && !out->pse) return fail(VPM_ERR_ARG, "vpm_batch_run: pse missing");
+  if ((want & VPM_OUT_VORT) && n && !out->vort) return fail(VPM_ERR_ARG, "vpm_batch_run: vort missing");
+  if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) return fail(VPM_ERR_ARG, "vpm_batch_run: enstrophy missing");
+  BatchLayout L{};
+  if ((rc = batch_layout(n, d->nsets, d->n_items, &L))) return rc;
+  if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_run: workspace too small");
+  hipStream_t st = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  double* scratch = reinterpret_cast<double*>(ws + L.scratch_vec);
+  BatchArgs b{};
+  auto route = [&](unsigned bit, double* dst, double** ptr, int64_t* ld) {
+    if (want & bit) {
+      *ptr = dst;
+      *ld = out->ld;
+    } else {
+      *ptr = scratch;
+      *ld = n > 0 ? n : 1;
+    }
+  };
+  route(VPM_OUT_VEL, out->vel, &b.f.vel, &b.f.ld_vel);
+  route(VPM_OUT_STRETCH, out->stretch, &b.f.str, &b.f.ld_str);
+  route(VPM_OUT_PSE, out->pse, &b.f.pse, &b.f.ld_pse);
+  route(VPM_OUT_VORT, out->vort, &b.f.vort, &b.f.ld_vort);
+  b.f.ens_part = reinterpret_cast<double*>(ws + L.ens_part);
+  b.src = reinterpret_cast<const Src8*>(ws + L.src8);
+  b.perm = reinterpret_cast<const int*>(ws + L.perm);
+  b.inst_off = d->set_off;
+  b.item_inst = d->item_set;
+  b.item_block = d->item_block;
+  b.n_items = d->n_items;
+  b.counter = reinterpret_cast<int*>(ws + L.counter);
+  if ((rc = dispatch_batch(b, batch_mask(want), o->precision, o->tile_config < 0 ? 0 : o->tile_config,
+                           d->grid, st)))
+    return rc;
+  if ((want & VPM_OUT_ENSTROPHY) && d->nsets > 0) {
+    const double k = (1.0 / (8.0 * kPi)) * kOneOverFourPi;
+    hipLaunchKernelGGL(set_enstrophy_kernel, dim3(d->nsets), dim3(256), 0, st, b.f.ens_part,
+                       d->set_item_first, d->set_item_count, k, out->enstrophy);
+    VPM_LAUNCH_CHECK("set_enstrophy_kernel");
+  }
+  return VPM_OK;
+}
+
+int vpm_batch_les_update(const vpm_batch_desc_t* d, int64_t ld, const double* vorticity,
+                         const double* radii, const double* enstrophy, double* state,
+                         const double* nu, double* nu_t, double* nu_eff, void* workspace,
+                         size_t ws_bytes, int64_t n_total, void* stream) {
+  clear_error();
+  int rc = check_desc(d, nullptr, "vpm_batch_les_update");
+  if (rc) return rc;
+  if (n_total < 0 || ld < n_total) return fail(VPM_ERR_ARG, "vpm_batch_les_update: bad n/ld");
+  BatchLayout L{};
+  if ((rc = batch_layout(n_total, d->nsets, d->n_items, &L))) return rc;
+  if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_les_update: workspace too small");
+  if (d->nsets == 0 || n_total == 0) return VPM_OK;
+  if (!vorticity || !radii || !enstrophy || !state || !nu || !nu_t || !nu_eff)
+    return fail(VPM_ERR_ARG, "vpm_batch_les_update: null pointer");
+  hipLaunchKernelGGL(set_les_kernel, dim3(d->nsets), dim3(256), 0, as_stream(stream), d->set_off, ld,
+                     vorticity, radii, enstrophy, state, nu, nu_t, nu_eff,
+                     reinterpret_cast<double*>(static_cast<char*>(workspace) + L.cand));
+  VPM_LAUNCH_CHECK("set_les_kernel");
+  return VPM_OK;
 }
 
 }  // extern "C"

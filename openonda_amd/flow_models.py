@@ -50,7 +50,7 @@ def torus_ring(n_stations=64, R=1.0, sigma_c=0.1, gamma=1.0, h=0.1, centre=(0.0,
     cs = cross_section(h)
     th = 2.0 * np.pi * np.arange(n_stations) / n_stations
     rho, psi, area = cs[:, 0], cs[:, 1], cs[:, 2]
-    rr = R + rho[None, :] * np.cos(psi)[None, :]              # (st, 64) distance to axis
+    rr = np.broadcast_to(R + rho * np.cos(psi), (n_stations, len(cs)))  # distance to the axis
     xa = np.broadcast_to(rho * np.sin(psi), rr.shape)
     y = rr * np.cos(th)[:, None]
     z = rr * np.sin(th)[:, None]

@@ -45,7 +45,7 @@ def _vec(t: torch.Tensor, name: str, n: int | None = None) -> torch.Tensor:
         raise ValueError(f"{name} must be on the HIP device")
     if t.dim() != 2 or t.shape[0] != 3:
         raise ValueError(f"{name} must have shape (3, n), got {tuple(t.shape)}")
-    if t.stride(1) != 1 or (t.shape[1] > 0 and t.stride(0) < t.shape[1]):
+    if (t.shape[1] > 1 and t.stride(1) != 1) or (t.shape[1] > 0 and t.stride(0) < t.shape[1]):
         raise ValueError(f"{name} must be row-contiguous SoA (stride (ld, 1))")
     if n is not None and t.shape[1] != n:
         raise ValueError(f"{name} has {t.shape[1]} columns, expected {n}")
@@ -65,7 +65,7 @@ def _scal(t: torch.Tensor, name: str, n: int) -> torch.Tensor:
 
 
 def _ld(t: torch.Tensor) -> int:
-    return t.stride(0) if t.shape[1] > 0 else max(1, t.shape[1])
+    return t.stride(0) if t.shape[1] > 0 else 1
 
 
 def _particles(pos, alpha, sigma, nu=None) -> L.Particles:

@@ -1,0 +1,150 @@
+"""GPU tests of the device-resident ParticleSystem against the reference-step oracle.
+
+oracle/vpm_steps.py restates vpmModule.py's update_velocities /
+update_strengths / update_state (including the strength-history bookkeeping
+and the LES filter) in FP64 NumPy; the HIP system must follow it step by step.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import vpm_oracle as O
+from oracle.vpm_steps import ReferenceStepper
+from openonda_amd import flow_models as FM
+from openonda_amd.Particle import Particle
+from openonda_amd.vpmModule import ParticleSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0)
+
+
+def cloud_with_mean(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 3))
+    a = rng.normal(0, 1, (n, 3)) / n + np.array([0.0, 0.0, 0.5]) / n
+    s = np.full(n, 0.25)
+    nu = np.full(n, 2e-3)
+    return x, a, s, nu
+
+
+CASES = [
+    # flow_model, viscous, method, unit, relax, layout
+    ("DNS", "PSE", "RK4", "GPU", False, "ring"),
+    ("DNS", "CoreSpreading", "RK3", "GPU", True, "ring"),
+    ("LES", "CoreSpreading", "RK2", "GPU", False, "cloud"),
+    ("LES", "PSE", "RK4", "CPU", False, "cloud"),
+    ("pseudo2D", "PSE", "Euler", "CPU", False, "ring"),
+    ("Potential", None, "RK4", "GPU", False, "ring"),
+    ("DNS", "PSE", "RK2", "CPU", False, "cloud"),
+]
+
+
+@pytest.mark.parametrize("flow,visc,method,unit,relax,layout", CASES)
+def test_steps_follow_reference(gpu, flow, visc, method, unit, relax, layout):
+    print(f"[system] {flow} {visc} {method} {unit} relax={relax} {layout}", flush=True)
+    if layout == "ring":
+        x, a, s, nu = FM.torus_ring(8, nu=1e-3)
+    else:
+        x, a, s, nu = cloud_with_mean(400, 7)
+    dt = 0.02
+    ref = ReferenceStepper(x, a, s, nu, dt=dt, method=method, flow_model=flow,
+                           viscous_scheme=visc, variant="cuda" if unit == "GPU" else "cpu",
+                           relax=relax)
+    ps = ParticleSystem(time_step_size=dt, time_integration_method=method, processing_unit=unit,
+                        flow_model=flow, viscous_scheme=visc, relax_strength_solution=relax,
+                        write_backups=False)
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
+    for k in range(5):
+        ref.step()
+        ps.update_velocities()
+        ps.update_strengths()
+        ps.update_state()
+        tag = (flow, visc, method, unit, k)
+        assert rel(ps.get_particle_positions(), ref.x) <= 1e-12, tag
+        assert rel(ps.get_particle_strengths(), ref.alpha) <= 1e-11, tag
+        assert rel(ps.get_particle_radii(), ref.sigma) <= 1e-14, tag
+        assert rel(ps.get_particle_velocities(), ref.velocity) <= 1e-11, tag
+        assert rel(ps.get_particle_viscosities_t(), ref.nu_t) <= 1e-10, tag
+        if flow == "LES":
+            assert ps.Cnu == pytest.approx(ref.cnu, rel=1e-12), tag
+    assert ps.time_step == 5 and ps.flow_time == pytest.approx(5 * dt)
+
+
+def test_fast_precisions_track_plain(gpu):
+    x, a, s, nu = FM.torus_ring(16, nu=1e-3)
+    runs = {}
+    for prec in ("plain", "f64", "f32c"):
+        ps = ParticleSystem(time_step_size=0.02, time_integration_method="RK4",
+                            processing_unit="GPU", flow_model="LES", viscous_scheme="PSE",
+                            precision=prec, write_backups=False)
+        ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
+        for _ in range(4):
+            ps.update_velocities()
+            ps.update_strengths()
+            ps.update_state()
+        runs[prec] = (ps.get_particle_positions(), ps.get_particle_strengths())
+    assert rel(runs["f64"][0], runs["plain"][0]) <= 1e-12
+    assert rel(runs["f64"][1], runs["plain"][1]) <= 1e-11
+    assert rel(runs["f32c"][0], runs["plain"][0]) <= 1e-6
+    assert rel(runs["f32c"][1], runs["plain"][1]) <= 1e-5
+
+
+def test_diagnostics_match_oracle(gpu):
+    x, a, s, nu = FM.torus_ring(16)
+    ps = ParticleSystem(processing_unit="GPU", write_backups=False)
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
+    assert rel(ps.get_total_linear_impulse(), O.total_linear_impulse(x, a)) <= 1e-13
+    assert rel(ps.get_total_angular_impulse(), O.total_angular_impulse(x, a) + 1e-300) <= 1e-10 or \
+        np.abs(ps.get_total_angular_impulse()).max() < 1e-14
+    assert ps.get_total_strength_magnitude() == pytest.approx(np.sum(np.linalg.norm(a, axis=1)), rel=1e-13)
+    assert ps.get_total_kinetic_energy() == pytest.approx(O.total_kinetic_energy(x, a, s), rel=1e-11)
+    assert ps.get_total_enstrophy() == pytest.approx(O.total_enstrophy(x, a, s), rel=1e-11)
+    y = FM.ring_probes(64)
+    assert rel(ps.get_induced_velocity_at(y), O.induced_velocity_at(y, x, a, s)) <= 1e-12
+    assert rel(ps.get_vorticity_field_at(y), O.vorticity_field_at(y, x, a, s)) <= 1e-12
+
+
+def test_particle_management_and_backup(gpu, tmp_path):
+    x, a, s, nu = FM.torus_ring(4, nu=1e-3)
+    ps = ParticleSystem(processing_unit="GPU", write_backups=False,
+                        backup_filename=str(tmp_path / "b"))
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu, group_id=3)
+    ps.add_particle(Particle(position=[5.0, 0, 0], strength=[0, 0, 1e-9], radius=0.2))
+    assert len(ps) == len(x) + 1
+    p = ps[len(x)]
+    assert p.radius == 0.2 and p.group_id == 0 and np.allclose(p.position, [5, 0, 0])
+    ps.backup_particle_field()
+    q = ParticleSystem(processing_unit="GPU", write_backups=False)
+    q.load_particle_field_from_backup(str(tmp_path / "b_0000000.vtp"))
+    assert np.array_equal(q.get_particle_positions(), ps.get_particle_positions())
+    assert np.array_equal(q.get_particle_strengths(), ps.get_particle_strengths())
+    assert np.array_equal(q.get_particle_group_id(), ps.get_particle_group_id())
+    total = ps.get_total_strength_magnitude()
+    ps.remove_weak_particles("relative", 1e-3, conserve_total_circulation=True)
+    assert len(ps) == len(x)
+    assert ps.get_total_strength_magnitude() == pytest.approx(total, rel=1e-12)
+    ps.remove_particles([0, 1, 1, 10 ** 6])
+    assert len(ps) == len(x) - 2
+    ps.remove_all_particles()
+    assert len(ps) == 0
+
+
+def test_constructor_surface(gpu):
+    # the reference's only VPM test: tests/test1_import_modules.py:23-58
+    ps = ParticleSystem(flow_model="LES", time_step_size=0.01, time_integration_method="RK2",
+                        viscous_scheme="CoreSpreading", processing_unit="GPU",
+                        monitor_variables=["Circulation", "Kinetic energy"],
+                        backup_filename="particle_system_backup.dat", backup_frequency=10)
+    assert ps.flow_model == "LES" and ps.time_step_size == 0.01
+    with pytest.raises(ValueError):
+        ParticleSystem(flow_model="Potential", viscous_scheme="PSE")
+    with pytest.raises(ValueError):
+        ParticleSystem(time_integration_method="RK5")
+    with pytest.raises(TypeError):
+        ParticleSystem(time_step_size="1")
