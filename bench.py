@@ -53,6 +53,10 @@ def main(argv=None):
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one forward step and one 256 MiB copy, for rocprofv3 --pmc runs")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -98,6 +102,15 @@ def main(argv=None):
                sh)
         if ev is not None:
             ev[1].record(stream)
+
+    if args.pmc_probe:
+        step()
+        src = torch.ones(1 << 25, dtype=torch.float64, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] pmc probe done: 1 fused step, 1 copy of 268435456 bytes")
+        return 0
 
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -201,6 +214,81 @@ def main(argv=None):
             "cpu_model": CB.host_cpu_model(),
         }
         log(f"[bench] CPU {pps:.3e} pairs/s ({secs:.1f} s); rel-L2 {rel}")
+
+    # ---- adjoint / forward ratio (differentiation workload size) -----------
+    if not args.skip_adjoint:
+        from openonda_amd import autograd as AG
+
+        xr, ar, sr, nur = FM.torus_ring(2048, nu=1e-3)  # N = 131072
+        nr = len(xr)
+        rp, ra = P.to_soa(xr, dev), P.to_soa(ar, dev)
+        rs, rn = torch.from_numpy(sr).to(dev), torch.from_numpy(nur).to(dev)
+        g = torch.Generator(device=dev).manual_seed(0)
+        cots = [torch.randn(3, nr, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+
+        def fwd():
+            P.forward(rp, ra, rs, rn, outputs=("vel", "stretch", "pse"), precision="f32c",
+                      sigma_mode="uniform")
+
+        def adj():
+            AG.adjoint(rp, ra, rs, rn, *cots, precision="f32c", sigma_mode="uniform")
+
+        times = {}
+        for name, fn in (("forward", fwd), ("adjoint", adj)):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            times[name] = e0.elapsed_time(e1) / 3
+        result["adjoint"] = {
+            "workload": f"ring, N={nr}, velocity+stretching+PSE, FP32-compensated",
+            "forward_ms": times["forward"], "adjoint_ms": times["adjoint"],
+            "adjoint_over_forward": times["adjoint"] / times["forward"]}
+        log(f"[bench] adjoint/forward = {times['adjoint']:.2f}/{times['forward']:.2f} ms "
+            f"= {times['adjoint'] / times['forward']:.2f}")
+
+    # ---- batched persistent launch, graph-replayed RK4 steps -----------------
+    if not args.skip_batch:
+        from openonda_amd import batch as B
+
+        sets = FM.batch_instances()
+        bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True)
+        log(f"[bench] batch: {bs.nsets} sets, N_total={bs.n}, {bs.n_items} work items")
+        bs.step()          # sorts, eager
+        bs.capture()
+        bs.replay(1)       # warm
+        bs.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(bs.stream)
+        bs.replay(args.batch_steps)
+        e1.record(bs.stream)
+        bs.synchronize()
+        bms = e0.elapsed_time(e1)
+        # per-set accuracy of one FP32-compensated evaluation against the plain FP64 kernels
+        bs.evaluate(bs.x, bs.a, 3, True, bs.stream.cuda_stream)
+        bs.synchronize()
+        errs = []
+        for k in range(bs.nsets):
+            sl = bs.set_slice(k)
+            xs, as_, ss = bs.x[:, sl], bs.a[:, sl], bs.sigma[sl]
+            u = P.selfinduced_velocity(xs, as_, ss)
+            sg = P.strength_gradients(xs, as_, ss)
+            errs.append(max(float((bs.vel[:, sl] - u).norm() / u.norm()),
+                            float((bs.str_[:, sl] - sg).norm() / sg.norm())))
+        pairs_step = 4 * bs.pairs_per_evaluation()
+        result["batch"] = {
+            "workload": f"{bs.nsets} ring/cloud sets, N_i log-uniform in [2048, 131072], "
+                        f"N_total={bs.n}; RK4 (velocity+stretching+PSE, LES filter per set), "
+                        f"one persistent launch per stage, HIP-graph replay",
+            "steps": args.batch_steps, "ms_per_step": bms / args.batch_steps,
+            "pairs_per_s": pairs_step * args.batch_steps / (bms * 1e-3),
+            "rel_l2_max_vs_fp64": max(errs), "rel_l2_median_vs_fp64": float(np.median(errs))}
+        log(f"[bench] batch: {bms / args.batch_steps:.1f} ms/step, "
+            f"{result['batch']['pairs_per_s']:.3e} pairs/s, max rel-L2 {max(errs):.2e}")
 
     print(json.dumps(result), flush=True)
     return 0

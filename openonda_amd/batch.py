@@ -1,0 +1,216 @@
+"""Many particle sets of uneven size advanced together on one GPU.
+
+Each RK4 stage of every set runs in ONE persistent launch (``vpm_batch_run``:
+workgroups pull (set, target block) items, largest set first), and a whole
+time step -- four stages, their axpys, the per-set LES filter and the final
+RK4 combination -- is captured once in a HIP graph and replayed
+(``vpm_graph_*``).  Sets keep a uniform radius each.
+
+The step integrates the coupled particle ODE with classical RK4:
+dx/dt = u(x, alpha), dalpha/dt = S(x, alpha) [+ D_PSE(x, alpha; nu + nu_t)].
+With ``les=True`` stage 1 also evaluates the smoothed vorticity and the
+enstrophy of every set and applies the dynamic LES filter of
+vpmModule.py:887-962 per set; the resulting nu_t enters the PSE term from
+stage 2 on (stage 1 of a step uses the nu_t of the previous step).
+
+The reference solver advances one set per call and has no batched path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import physics as P
+
+
+def work_list(sizes, targets_per_item):
+    """(item_set, item_block, set_item_first, set_item_count): sets ordered by
+    decreasing size, the target blocks of a set consecutive."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    order = np.argsort(-sizes, kind="stable")
+    item_set, item_block = [], []
+    first = np.zeros(len(sizes), dtype=np.int32)
+    count = np.zeros(len(sizes), dtype=np.int32)
+    for s in order:
+        nb = int(math.ceil(sizes[s] / targets_per_item))
+        first[s] = len(item_set)
+        count[s] = nb
+        item_set.extend([s] * nb)
+        item_block.extend(range(nb))
+    return (np.array(item_set, dtype=np.int32), np.array(item_block, dtype=np.int32), first, count)
+
+
+class BatchedSystem:
+    """Particle sets [(x (N_i,3), alpha (N_i,3), sigma (N_i,), nu (N_i,)), ...]."""
+
+    def __init__(self, sets, dt, precision="f32c", tile_config=-1, pse=True, les=True,
+                 device=None):
+        P.require_gpu()
+        if precision not in ("f32c", "f64"):
+            raise ValueError("precision must be 'f32c' or 'f64'")
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        self.dt = float(dt)
+        self.precision = precision
+        self.pse = pse
+        self.les = les
+        self.tile_config = int(tile_config)
+        sizes = [len(s[0]) for s in sets]
+        for k, s in enumerate(sets):
+            sig = np.asarray(s[2])
+            if len(sig) and not np.all(sig == sig[0]):
+                raise ValueError(f"set {k}: the batched kernels need a uniform radius per set")
+        self.sizes = np.array(sizes, dtype=np.int64)
+        self.nsets = len(sets)
+        n = int(self.sizes.sum())
+        self.n = n
+        if n >= 2 ** 31:
+            raise ValueError("too many particles")
+        d = self.device
+        cat = lambda i: np.concatenate([np.asarray(s[i], dtype=np.float64) for s in sets])
+        self.x = P.to_soa(cat(0).reshape(-1, 3), d)
+        self.a = P.to_soa(cat(1).reshape(-1, 3), d)
+        self.sigma = torch.from_numpy(cat(2)).to(d)
+        self.nu = torch.from_numpy(cat(3)).to(d)
+        self.nu_t = torch.zeros_like(self.nu)
+        self.nu_eff = self.nu.clone()
+        off = np.zeros(self.nsets + 1, dtype=np.int32)
+        off[1:] = np.cumsum(self.sizes)
+        self.offsets = off
+        if precision == "f64":
+            tpi = 256
+        else:
+            cfgs = P.tile_configs()
+            b, r = cfgs[self.tile_config if self.tile_config >= 0 else 0][:2]
+            tpi = b * r
+        item_set, item_block, first, count = work_list(self.sizes, tpi)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(d)
+        self._desc_t = [t(off), t(np.repeat(np.arange(self.nsets), self.sizes)), t(item_set),
+                        t(item_block), t(first), t(count)]
+        self.n_items = len(item_set)
+        self.desc = L.BatchDesc(self.nsets, self.n_items, 0, *[x.data_ptr() for x in self._desc_t])
+        lib = L.load()
+        wsb = lib.vpm_batch_workspace_bytes(n, self.nsets, self.n_items)
+        if wsb == 0:
+            raise L.VpmError("vpm_batch_workspace_bytes", -3, lib.vpm_last_error().decode())
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+        self.wsb = wsb
+        e = lambda: torch.zeros((3, n), dtype=torch.float64, device=d)
+        self.kx = [e() for _ in range(4)]
+        self.ka = [e() for _ in range(4)]
+        self.xs, self.as_ = e(), e()
+        self.vel, self.str_, self.pse_, self.vort = e(), e(), e(), e()
+        self.enstrophy = torch.zeros(max(self.nsets, 1), dtype=torch.float64, device=d)
+        st = np.zeros((max(self.nsets, 1), 3))
+        st[:, 0] = 0.02958                                      # vpmModule.py:92
+        self.les_state = torch.from_numpy(st).to(d)
+        self.stream = torch.cuda.Stream(device=d)
+        self.graph = None
+        self.steps_done = 0
+        self._sorted = False
+
+    # ------------------------------------------------------------ primitives
+    def _parts(self, x, a):
+        return L.Particles(self.n, max(self.n, 1), x.data_ptr(), a.data_ptr(),
+                           self.sigma.data_ptr(), self.nu_eff.data_ptr())
+
+    def _mask(self, stage1):
+        m = L.OUT_VEL | L.OUT_STRETCH
+        if self.pse:
+            m |= L.OUT_PSE
+        if stage1 and self.les:
+            m |= L.OUT_VORT | L.OUT_ENSTROPHY
+        return m
+
+    def evaluate(self, x, a, mask, resort, stream, vel=None, stretch=None):
+        """prepare + one persistent launch.  Outputs go to ``vel``/``stretch``
+        (default self.vel/self.str_), self.pse_, self.vort, self.enstrophy."""
+        p = self._parts(x, a)
+        L.call("vpm_batch_prepare", C.byref(p), C.byref(self.desc), 1 if resort else 0,
+               self.ws.data_ptr(), self.wsb, stream)
+        o = L.ForwardOpts(mask, P.PRECISIONS[self.precision], L.SIGMA_UNIFORM, 0, self.tile_config)
+        vel = self.vel if vel is None else vel
+        stretch = self.str_ if stretch is None else stretch
+        fo = L.ForwardOut(max(self.n, 1), vel.data_ptr(), stretch.data_ptr(),
+                          self.pse_.data_ptr(), self.vort.data_ptr(), self.enstrophy.data_ptr())
+        L.call("vpm_batch_run", C.byref(p), C.byref(self.desc), C.byref(o), C.byref(fo),
+               self.ws.data_ptr(), self.wsb, stream)
+
+    def _axpy(self, x, y, c, out, stream):
+        L.call("vpm_axpy3", self.n, max(self.n, 1), x.data_ptr(), y.data_ptr(), float(c),
+               out.data_ptr(), stream)
+
+    def _stage(self, k, x, a, resort, stream):
+        # kx = u, ka = S (+ D)
+        self.evaluate(x, a, self._mask(k == 0), resort, stream, self.kx[k], self.ka[k])
+        if self.pse:
+            self._axpy(self.ka[k], self.pse_, 1.0, self.ka[k], stream)
+        if k == 0 and self.les:
+            L.call("vpm_batch_les_update", C.byref(self.desc), max(self.n, 1),
+                   self.vort.data_ptr(), self.sigma.data_ptr(), self.enstrophy.data_ptr(),
+                   self.les_state.data_ptr(), self.nu.data_ptr(), self.nu_t.data_ptr(),
+                   self.nu_eff.data_ptr(), self.ws.data_ptr(), self.wsb, self.n, stream)
+
+    def _step(self, resort, stream):
+        dt = self.dt
+        self._stage(0, self.x, self.a, resort, stream)
+        for k, h in ((1, 0.5 * dt), (2, 0.5 * dt), (3, dt)):
+            self._axpy(self.x, self.kx[k - 1], h, self.xs, stream)
+            self._axpy(self.a, self.ka[k - 1], h, self.as_, stream)
+            self._stage(k, self.xs, self.as_, False, stream)
+        for y, ks in ((self.x, self.kx), (self.a, self.ka)):
+            L.call("vpm_rk4_combine", self.n, max(self.n, 1), y.data_ptr(), ks[0].data_ptr(),
+                   ks[1].data_ptr(), ks[2].data_ptr(), ks[3].data_ptr(), dt, y.data_ptr(), stream)
+
+    # ------------------------------------------------------------ public
+    def step(self, resort=None):
+        """One eager step on the system's stream (sorts on the first call)."""
+        resort = (not self._sorted) if resort is None else resort
+        torch.cuda.current_stream(self.device).synchronize()
+        self._step(resort, self.stream.cuda_stream)
+        self._sorted = True
+        self.steps_done += 1
+
+    def capture(self):
+        """Captures one step (without the sort) in a HIP graph."""
+        if not self._sorted:
+            raise RuntimeError("run one eager step (it sorts the sets) before capturing")
+        s = self.stream.cuda_stream
+        self.stream.synchronize()
+        L.call("vpm_graph_begin", s)
+        try:
+            self._step(False, s)
+        finally:
+            g = C.c_void_p()
+            rc = L.load().vpm_graph_end(s, C.byref(g))
+        if rc != 0:
+            raise L.VpmError("vpm_graph_end", rc, L.load().vpm_last_error().decode())
+        if self.graph is not None:
+            L.call("vpm_graph_destroy", self.graph)
+        self.graph = g.value
+
+    def replay(self, steps=1):
+        if self.graph is None:
+            self.capture()
+        for _ in range(steps):
+            L.call("vpm_graph_launch", self.graph, self.stream.cuda_stream)
+            self.steps_done += 1
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def set_slice(self, s):
+        return slice(int(self.offsets[s]), int(self.offsets[s + 1]))
+
+    def pairs_per_evaluation(self):
+        return float(np.sum(self.sizes.astype(np.float64) ** 2))
+
+    def __del__(self):
+        try:
+            if self.graph is not None:
+                L.load().vpm_graph_destroy(self.graph)
+        except Exception:
+            pass

@@ -515,7 +515,9 @@ struct WsLayout {
   size_t cub_bytes;
 };
 
-int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L) {
+// The hipCUB scratch comes last, so the other offsets do not depend on it;
+// calls that do not sort pass need_cub = false and skip the size query.
+int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = true) {
   const size_t N = static_cast<size_t>(n > 0 ? n : 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -534,7 +536,7 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L) {
   // Unrequested outputs of a superset kernel land here: 3 vectors of N.
   L->scratch_vec = take(3 * N * sizeof(double));
   size_t cub_bytes = 0;
-  if (!o || !o->presorted) {
+  if (need_cub && (!o || !o->presorted)) {
     const int rc = sort_temp_bytes(n, &cub_bytes);
     if (rc) return rc;
   }
@@ -771,7 +773,7 @@ struct BatchLayout {
       total, cub_bytes;
 };
 
-int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L) {
+int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L, bool need_cub = true) {
   const size_t N = static_cast<size_t>(n > 0 ? n : 1);
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -790,13 +792,15 @@ int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L) {
   L->scratch_vec = take(3 * N * sizeof(double));
   L->counter = take(64);
   size_t cb = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
-      nullptr, cb, static_cast<const uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
-      static_cast<const int*>(nullptr), static_cast<int*>(nullptr), static_cast<int>(N), 0, 62,
-      hipStream_t(0));
-  if (e != hipSuccess) {
-    set_error("hipcub SortPairs size query failed: %s", hipGetErrorString(e));
-    return VPM_ERR_LAUNCH;
+  if (need_cub) {
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, cb, static_cast<const uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+        static_cast<const int*>(nullptr), static_cast<int*>(nullptr), static_cast<int>(N), 0, 62,
+        hipStream_t(0));
+    if (e != hipSuccess) {
+      set_error("hipcub SortPairs size query failed: %s", hipGetErrorString(e));
+      return VPM_ERR_LAUNCH;
+    }
   }
   L->cub_bytes = cb;
   L->cub = take(cb + 16);
@@ -877,8 +881,8 @@ int check_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, const c
 }
 
 int check_ws(int64_t n, const vpm_forward_opts_t* o, void* ws, size_t ws_bytes, WsLayout* L,
-             const char* who) {
-  int rc = layout(n, o, L);
+             const char* who, bool need_cub) {
+  int rc = layout(n, o, L, need_cub);
   if (rc) return rc;
   if (ws_bytes < L->total || !ws) {
     set_error("%s: workspace %zu bytes < required %zu", who, ws_bytes, L->total);
@@ -896,7 +900,7 @@ int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, v
   if (rc) return rc;
   const int64_t n = p->n;
   WsLayout L{};
-  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_prepare"))) return rc;
+  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_prepare", true))) return rc;
   if (n == 0) return VPM_OK;
   hipStream_t st = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
@@ -932,7 +936,7 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   if ((want & VPM_OUT_VORT) && n && !out->vort) return fail(VPM_ERR_ARG, "vpm_forward_run: vort output missing");
   if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) return fail(VPM_ERR_ARG, "vpm_forward_run: enstrophy output missing");
   WsLayout L{};
-  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_run"))) return rc;
+  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_run", false))) return rc;
   hipStream_t st = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
   if (n == 0) {
@@ -997,7 +1001,7 @@ int vpm_batch_prepare(const vpm_particles_t* p, const vpm_batch_desc_t* d, int r
   int rc = check_desc(d, p, "vpm_batch_prepare");
   if (rc) return rc;
   BatchLayout L{};
-  if ((rc = batch_layout(p->n, d->nsets, d->n_items, &L))) return rc;
+  if ((rc = batch_layout(p->n, d->nsets, d->n_items, &L, resort != 0))) return rc;
   if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_prepare: workspace too small");
   if (p->n == 0) return VPM_OK;
   hipStream_t st = as_stream(stream);
@@ -1051,7 +1055,7 @@ int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm
   if ((want & VPM_OUT_VORT) && n && !out->vort) return fail(VPM_ERR_ARG, "vpm_batch_run: vort missing");
   if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) return fail(VPM_ERR_ARG, "vpm_batch_run: enstrophy missing");
   BatchLayout L{};
-  if ((rc = batch_layout(n, d->nsets, d->n_items, &L))) return rc;
+  if ((rc = batch_layout(n, d->nsets, d->n_items, &L, false))) return rc;
   if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_run: workspace too small");
   hipStream_t st = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
@@ -1099,7 +1103,7 @@ int vpm_batch_les_update(const vpm_batch_desc_t* d, int64_t ld, const double* vo
   if (rc) return rc;
   if (n_total < 0 || ld < n_total) return fail(VPM_ERR_ARG, "vpm_batch_les_update: bad n/ld");
   BatchLayout L{};
-  if ((rc = batch_layout(n_total, d->nsets, d->n_items, &L))) return rc;
+  if ((rc = batch_layout(n_total, d->nsets, d->n_items, &L, false))) return rc;
   if (!workspace || ws_bytes < L.total) return fail(VPM_ERR_WORKSPACE, "vpm_batch_les_update: workspace too small");
   if (d->nsets == 0 || n_total == 0) return VPM_OK;
   if (!vorticity || !radii || !enstrophy || !state || !nu || !nu_t || !nu_eff)

@@ -20,23 +20,20 @@ constexpr int kB = 256;
 
 inline unsigned grid_for(int64_t n) { return static_cast<unsigned>(ceil_div(n > 0 ? n : 1, kB)); }
 
-__global__ __launch_bounds__(kB) void axpy3_kernel(int n, int64_t ld, const double* __restrict__ x,
-                                                   const double* __restrict__ y, double c,
-                                                   double* __restrict__ out) {
+// x, y and out may alias (in-place updates).
+__global__ __launch_bounds__(kB) void axpy3_kernel(int n, int64_t ld, const double* x,
+                                                   const double* y, double c, double* out) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
   for (int k = 0; k < 3; ++k) out[k * ld + i] = x[k * ld + i] + c * y[k * ld + i];
 }
 
-// out = x0 + dt/6 (k1 + 2 k2 + 2 k3 + k4)   (vpmModule.py:688)
-__global__ __launch_bounds__(kB) void rk4_combine_kernel(int n, int64_t ld,
-                                                         const double* __restrict__ x0,
-                                                         const double* __restrict__ k1,
-                                                         const double* __restrict__ k2,
-                                                         const double* __restrict__ k3,
-                                                         const double* __restrict__ k4, double dt,
-                                                         double* __restrict__ out) {
+// out = x0 + dt/6 (k1 + 2 k2 + 2 k3 + k4)   (vpmModule.py:688); out may be x0.
+__global__ __launch_bounds__(kB) void rk4_combine_kernel(int n, int64_t ld, const double* x0,
+                                                         const double* k1, const double* k2,
+                                                         const double* k3, const double* k4,
+                                                         double dt, double* out) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
