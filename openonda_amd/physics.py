@@ -204,6 +204,8 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
     tensor.  ``precision`` is "f32c" (FP32 pairs, compensated FP64 totals)
     or "f64".  ``sigma_mode`` "uniform" asserts sigma_i == sigma_0 (not
     checked), "general" uses sigma_ij per pair, "auto" checks on the device.
+    ``tile_config``: -1 (default shape), an index of :func:`tile_configs`, or
+    "auto" (the autotuner's cached winner for this N).
     ``out`` may supply preallocated tensors by name.
     """
     if isinstance(outputs, str):
@@ -225,6 +227,11 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
         sigma_mode = "uniform" if sigma_is_uniform(sigma) else "general"
     if sigma_mode not in ("uniform", "general"):
         raise ValueError("sigma_mode must be 'auto', 'uniform' or 'general'")
+    if tile_config == "auto":
+        from . import autotune
+
+        tile_config = autotune.best_config(n, outputs, precision, sigma_mode) \
+            if precision == "f32c" else -1
     opts = L.ForwardOpts(mask, PRECISIONS[precision],
                          L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL,
                          1 if presorted else 0, int(tile_config))

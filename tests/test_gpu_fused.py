@@ -156,3 +156,16 @@ def test_bad_arguments(gpu):
     from openonda_amd import _lib as L
     with pytest.raises(L.VpmError):
         P.forward(x, a, s, nu, tile_config=99)
+
+
+def test_autotune_records_a_valid_winner(gpu, tmp_path):
+    from openonda_amd import autotune as AT
+
+    x, a, s, nu = FM.random_cloud(1 << 14, seed=12)
+    d = to_dev(x, a, s, nu)
+    path = tmp_path / "cache.json"
+    best, report = AT.tune(d[0], d[1], d[2], outputs=("vel", "stretch"), path=path, reps=1)
+    assert len(report) == len(P.tile_configs())
+    assert all(r["rel_l2"] <= F32C_TOL for r in report)
+    assert AT.best_config(1 << 14, ("vel", "stretch"), path=path) == best["config"]
+    assert AT.best_config(1 << 20, ("vel", "stretch"), path=path) == -1
