@@ -19,8 +19,9 @@
 //           dX_m += C [(f (a_j.P_m + a_m.Q_j) - 2 h psi phi) r + f phi (a_j x a_m) + f psi d_mu]
 // PSE       dA_m += 105 C p (z_j - z_m)
 //           dX_m += -2 * 105 C q ((a_j - a_m).(z_m - z_j)) r
-// Sums over j of g a_j, g lam_j, f phi a_j, f psi and f psi mu_j are
-// accumulated per target and crossed with the target's vectors once.
+// Sums over j of g a_j, g lam_j and f phi a_j are accumulated per target and
+// crossed with the target's vectors once; the three radial terms share one
+// accumulator.
 // The same block-centred FP32 offsets, LDS tiles and FP32-partial / FP64-total
 // accumulation as the forward pass are used (precision F32C); F64 runs the
 // same pass in double.
@@ -100,127 +101,114 @@ struct AdjTgt {
   Real x, y, z, ax, ay, az, lx, ly, lz, mx, my, mz, px, py, pz, zx, zy, zz, hs;
 };
 
-// accumulator slots
-enum : int {
-  aA1 = 0,   // sum g (lam_j x r)
-  aX1 = 3,   // sum f (e1 - e2) r
-  aG = 6,    // sum g a_j
-  aL = 9,    // sum g lam_j
-  aB1 = 12,  // sum g Q_j
-  aB2 = 15,  // sum f phi c
-  aX2 = 18,  // sum (f (d_mu.w) - 2 h psi phi) r
-  aY1 = 21,  // sum f phi a_j
-  aY2 = 24,  // sum f psi mu_j
-  aS1 = 27,  // sum f psi
-  aP1 = 28,  // sum p (z_j - z_m)
-  aX3 = 31,  // sum q (d_a . d_z) r
-  aN = 34
-};
+// Accumulator slots (per target).  Terms with the same sigma scaling share
+// an accumulator; sc = 1/sigma with a uniform radius (offsets pre-scaled),
+// 1 otherwise (per-pair 1/sigma_ij powers folded into g, f, h, p, q):
+//   aG = sum g a_j, aL = sum g lam_j                      (crossed in the epilogue)
+//   aA = sum g (lam_j x r) + 105 p (z_j - z_m)            dA, scale C/s^2
+//   aB = sum f phi c - g Q_j                              dA, scale C/s^3
+//   aX = sum [f e + sc (f dw - 2 h psi phi) + 945 q dd] r dX, scale C/s^3
+//        with dd = (a_j - a_m).(z_j - z_m)
+//   aY = sum f phi a_j,  aD = sum f psi d_mu              dX, scale C/s^4
+enum : int { aG = 0, aL = 3, aA = 6, aB = 9, aX = 12, aY = 15, aD = 18, aN = 21 };
 
 template <unsigned OUT>
 struct AdjSlots {
   static constexpr bool vel = OUT & VPM_OUT_VEL, str = OUT & VPM_OUT_STRETCH,
                         pse = OUT & VPM_OUT_PSE;
   static __device__ constexpr bool used(int k) {
-    return (vel && (k < 12)) || (str && ((k >= 6 && k < 9) || (k >= 12 && k < 28))) ||
-           (pse && k >= 28);
+    return (k < 3 && (vel || str)) || (k >= 3 && k < 6 && vel) ||
+           (k >= 6 && k < 9 && (vel || pse)) || (k >= 9 && k < 12 && str) ||
+           (k >= 12 && k < 15) || (k >= 15 && str);
   }
 };
 
 template <typename Real, unsigned OUT, bool UNIF>
-__device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const AdjSrc<Real>& s) {
+__device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const AdjSrc<Real>& s,
+                                         Real sc) {
   using S = AdjSlots<OUT>;
   using M = Ar<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
-  const Real r2 = dx * dx + dy * dy + dz * dz;
-  Real rho2, inv = Real(1), inv2 = Real(1);
+  Real A, inv = Real(1), inv2 = Real(1);
   if constexpr (UNIF) {
-    rho2 = r2;
+    A = dz * dz + (dy * dy + (dx * dx + Real(1)));
   } else {
     inv = M::rcp(t.hs + s.hs);
     inv2 = inv * inv;
-    rho2 = r2 * inv2;
+    A = (dx * dx + dy * dy + dz * dz) * inv2 + Real(1);
   }
-  const Real t1 = M::rsq(rho2 + Real(1));
+  const Real t1 = M::rsq(A);
   const Real t2 = t1 * t1;
   const Real t5 = t2 * t2 * t1;
   const Real t7 = t5 * t2;
-  const Real t9 = t7 * t2;
+  Real xc = Real(0);  // radial coefficient of aX
   if constexpr (S::vel || S::str) {
-    Real g = (rho2 + Real(2.5)) * t5;
-    Real f = (Real(3) * rho2 + Real(10.5)) * t7;
+    Real g = (A + Real(1.5)) * t5;                   // (rho^2 + 2.5) t^5
+    Real f = (Real(3) * A + Real(7.5)) * t7;         // 3 (rho^2 + 3.5) t^7
     if constexpr (!UNIF) {
       const Real inv3 = inv2 * inv;
       g *= inv3;
       f *= inv3 * inv2;
     }
-    const Real cx = dy * s.az - dz * s.ay;   // c = r x a_j
+    const Real cx = dy * s.az - dz * s.ay;            // c = r x a_j
     const Real cy = dz * s.ax - dx * s.az;
     const Real cz = dx * s.ay - dy * s.ax;
     acc[aG + 0] += g * s.ax;
     acc[aG + 1] += g * s.ay;
     acc[aG + 2] += g * s.az;
     if constexpr (S::vel) {
-      const Real kx = s.ly * dz - s.lz * dy;  // lam_j x r
+      const Real kx = s.ly * dz - s.lz * dy;          // lam_j x r
       const Real ky = s.lz * dx - s.lx * dz;
       const Real kz = s.lx * dy - s.ly * dx;
-      acc[aA1 + 0] += g * kx;
-      acc[aA1 + 1] += g * ky;
-      acc[aA1 + 2] += g * kz;
-      const Real e = (t.lx * cx + t.ly * cy + t.lz * cz) - (t.ax * kx + t.ay * ky + t.az * kz);
-      const Real fe = f * e;
-      acc[aX1 + 0] += fe * dx;
-      acc[aX1 + 1] += fe * dy;
-      acc[aX1 + 2] += fe * dz;
+      acc[aA + 0] += g * kx;
+      acc[aA + 1] += g * ky;
+      acc[aA + 2] += g * kz;
       acc[aL + 0] += g * s.lx;
       acc[aL + 1] += g * s.ly;
       acc[aL + 2] += g * s.lz;
+      const Real e = (t.lx * cx + t.ly * cy + t.lz * cz) - (t.ax * kx + t.ay * ky + t.az * kz);
+      xc = f * e;
     }
     if constexpr (S::str) {
-      Real h = (Real(7.5) * rho2 + Real(33.75)) * t9;
+      Real h = (Real(7.5) * A + Real(26.25)) * (t7 * t2);  // 7.5 (rho^2 + 4.5) t^9
       if constexpr (!UNIF) h *= inv2 * inv2 * inv2 * inv;
       const Real dmx = t.mx - s.mx, dmy = t.my - s.my, dmz = t.mz - s.mz;
       const Real phi = dmx * dx + dmy * dy + dmz * dz;
       const Real psi = t.ax * cx + t.ay * cy + t.az * cz;
       const Real dw = (s.ax * t.px + s.ay * t.py + s.az * t.pz) +
                       (t.ax * s.qx + t.ay * s.qy + t.az * s.qz);
-      acc[aB1 + 0] += g * s.qx;
-      acc[aB1 + 1] += g * s.qy;
-      acc[aB1 + 2] += g * s.qz;
       const Real fphi = f * phi;
-      acc[aB2 + 0] += fphi * cx;
-      acc[aB2 + 1] += fphi * cy;
-      acc[aB2 + 2] += fphi * cz;
-      const Real rad = f * dw - Real(2) * h * psi * phi;
-      acc[aX2 + 0] += rad * dx;
-      acc[aX2 + 1] += rad * dy;
-      acc[aX2 + 2] += rad * dz;
-      acc[aY1 + 0] += fphi * s.ax;
-      acc[aY1 + 1] += fphi * s.ay;
-      acc[aY1 + 2] += fphi * s.az;
+      acc[aB + 0] += fphi * cx - g * s.qx;
+      acc[aB + 1] += fphi * cy - g * s.qy;
+      acc[aB + 2] += fphi * cz - g * s.qz;
+      acc[aY + 0] += fphi * s.ax;
+      acc[aY + 1] += fphi * s.ay;
+      acc[aY + 2] += fphi * s.az;
       const Real fpsi = f * psi;
-      acc[aS1] += fpsi;
-      acc[aY2 + 0] += fpsi * s.mx;
-      acc[aY2 + 1] += fpsi * s.my;
-      acc[aY2 + 2] += fpsi * s.mz;
+      acc[aD + 0] += fpsi * dmx;
+      acc[aD + 1] += fpsi * dmy;
+      acc[aD + 2] += fpsi * dmz;
+      xc += sc * (f * dw - Real(2) * h * psi * phi);
     }
   }
   if constexpr (S::pse) {
-    Real p = t9, q = t9 * t2;
+    Real p = t7 * t2, q = p * t2;                     // t^9, t^11
     if constexpr (!UNIF) {
       p *= inv2;
       q *= inv2 * inv2;
     }
-    acc[aP1 + 0] += p * (s.zx - t.zx);
-    acc[aP1 + 1] += p * (s.zy - t.zy);
-    acc[aP1 + 2] += p * (s.zz - t.zz);
-    const Real dd = (s.ax - t.ax) * (t.zx - s.zx) + (s.ay - t.ay) * (t.zy - s.zy) +
-                    (s.az - t.az) * (t.zz - s.zz);
-    const Real qd = q * dd;
-    acc[aX3 + 0] += qd * dx;
-    acc[aX3 + 1] += qd * dy;
-    acc[aX3 + 2] += qd * dz;
+    const Real zx = s.zx - t.zx, zy = s.zy - t.zy, zz = s.zz - t.zz;
+    const Real p105 = Real(105) * p;
+    acc[aA + 0] += p105 * zx;
+    acc[aA + 1] += p105 * zy;
+    acc[aA + 2] += p105 * zz;
+    // (a_j - a_m).(z_m - z_j) = -(a_j - a_m).dz
+    const Real dd = (s.ax - t.ax) * zx + (s.ay - t.ay) * zy + (s.az - t.az) * zz;
+    xc += Real(945) * q * dd;
   }
+  acc[aX + 0] += xc * dx;
+  acc[aX + 1] += xc * dy;
+  acc[aX + 2] += xc * dz;
 }
 
 struct AdjArgs {
@@ -300,6 +288,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
   t.zx = static_cast<Real>(zx); t.zy = static_cast<Real>(zy); t.zz = static_cast<Real>(zz);
   t.hs = static_cast<Real>(0.5 * sg);
 
+  const Real rsc = static_cast<Real>(scale);
   Real part[aN];
   double tot[aN];
 #pragma unroll
@@ -344,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     if (valid) {
       for (int sb = 0; sb < TILE; sb += SUB) {
 #pragma unroll 4
-        for (int q = 0; q < SUB; ++q) adj_pair<Real, OUT, UNIF>(part, t, lds[sb + q]);
+        for (int q = 0; q < SUB; ++q) adj_pair<Real, OUT, UNIF>(part, t, lds[sb + q], rsc);
         if constexpr (kF32) {
 #pragma unroll
           for (int q = 0; q < aN; ++q) {
@@ -371,39 +360,26 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     s4 = s2 * s2;
   }
   const double* T = tot;
-  double dA[3] = {0, 0, 0}, dX[3] = {0, 0, 0};
+  double dA[3], dX[3];
+  for (int c = 0; c < 3; ++c) {
+    dA[c] = C * s2 * T[aA + c];
+    dX[c] = C * s3 * T[aX + c];
+  }
   const double G[3] = {T[aG], T[aG + 1], T[aG + 2]};
-  if constexpr (S::vel) {
+  if constexpr (S::vel) {  // - C/s^3 (G x lam_m - a_m x Lambda)
     const double Lm[3] = {T[aL], T[aL + 1], T[aL + 2]};
-    // G x lam_m - a_m x Lambda
-    const double gx = G[1] * lz - G[2] * ly - (ay * Lm[2] - az * Lm[1]);
-    const double gy = G[2] * lx - G[0] * lz - (az * Lm[0] - ax * Lm[2]);
-    const double gz = G[0] * ly - G[1] * lx - (ax * Lm[1] - ay * Lm[0]);
-    for (int c = 0; c < 3; ++c) dA[c] += C * s2 * T[aA1 + c];
-    dX[0] += C * s3 * (T[aX1] - gx);
-    dX[1] += C * s3 * (T[aX1 + 1] - gy);
-    dX[2] += C * s3 * (T[aX1 + 2] - gz);
+    dX[0] -= C * s3 * ((G[1] * lz - G[2] * ly) - (ay * Lm[2] - az * Lm[1]));
+    dX[1] -= C * s3 * ((G[2] * lx - G[0] * lz) - (az * Lm[0] - ax * Lm[2]));
+    dX[2] -= C * s3 * ((G[0] * ly - G[1] * lx) - (ax * Lm[1] - ay * Lm[0]));
   }
-  if constexpr (S::str) {
-    // dA: C/s^3 (G x mu_m - B1 + B2)
-    const double gmx = G[1] * mz - G[2] * my, gmy = G[2] * mx - G[0] * mz,
-                 gmz = G[0] * my - G[1] * mx;
-    dA[0] += C * s3 * (gmx - T[aB1] + T[aB2]);
-    dA[1] += C * s3 * (gmy - T[aB1 + 1] + T[aB2 + 1]);
-    dA[2] += C * s3 * (gmz - T[aB1 + 2] + T[aB2 + 2]);
-    // dX: C/s^4 (X2 + Y1 x a_m + mu_m S1 - Y2)
-    const double Y1[3] = {T[aY1], T[aY1 + 1], T[aY1 + 2]};
-    const double yx = Y1[1] * az - Y1[2] * ay, yy = Y1[2] * ax - Y1[0] * az,
-                 yz = Y1[0] * ay - Y1[1] * ax;
-    dX[0] += C * s4 * (T[aX2] + yx + mx * T[aS1] - T[aY2]);
-    dX[1] += C * s4 * (T[aX2 + 1] + yy + my * T[aS1] - T[aY2 + 1]);
-    dX[2] += C * s4 * (T[aX2 + 2] + yz + mz * T[aS1] - T[aY2 + 2]);
-  }
-  if constexpr (S::pse) {
-    for (int c = 0; c < 3; ++c) {
-      dA[c] += 105.0 * C * s2 * T[aP1 + c];
-      dX[c] += -2.0 * 105.0 * 4.5 * C * s3 * T[aX3 + c];
-    }
+  if constexpr (S::str) {  // dA += C/s^3 (G x mu_m + B);  dX += C/s^4 (Y x a_m + D)
+    dA[0] += C * s3 * ((G[1] * mz - G[2] * my) + T[aB]);
+    dA[1] += C * s3 * ((G[2] * mx - G[0] * mz) + T[aB + 1]);
+    dA[2] += C * s3 * ((G[0] * my - G[1] * mx) + T[aB + 2]);
+    const double Y[3] = {T[aY], T[aY + 1], T[aY + 2]};
+    dX[0] += C * s4 * ((Y[1] * az - Y[2] * ay) + T[aD]);
+    dX[1] += C * s4 * ((Y[2] * ax - Y[0] * az) + T[aD + 1]);
+    dX[2] += C * s4 * ((Y[0] * ay - Y[1] * ax) + T[aD + 2]);
   }
   const int64_t o = a.perm ? a.perm[k] : k;
   for (int c = 0; c < 3; ++c) {
@@ -412,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
   }
 }
 
-constexpr int kAdjBlock = 256, kAdjTile = 256, kAdjSub = 16;
+constexpr int kAdjBlock = 256, kAdjTile = 256, kAdjSub = 32;
 
 template <typename Real, unsigned OUT, bool UNIF>
 int launch_adj(const AdjArgs& a, hipStream_t st) {

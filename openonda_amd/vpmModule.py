@@ -146,12 +146,14 @@ class ParticleSystem:
         self._nut = torch.zeros((n,), dtype=f, device=d)
         self._gid = torch.zeros((n,), dtype=torch.int64, device=d)
         self._hist = None
+        self._unif = None
 
     def _set_state(self, pos, vel, st, sig, nu, nut, gid):
         self._pos, self._vel, self._str = pos.contiguous(), vel.contiguous(), st.contiguous()
         self._sigma, self._nu, self._nut, self._gid = (sig.contiguous(), nu.contiguous(),
                                                        nut.contiguous(), gid.contiguous())
         self._hist = None  # strength history refers to the old particle set
+        self._unif = None
 
     @property
     def n(self) -> int:
@@ -277,7 +279,10 @@ class ParticleSystem:
 
     # ------------------------------------------------------------ pair sums
     def _sigma_mode(self):
-        return "uniform" if P.sigma_is_uniform(self._sigma) else "general"
+        """Uniform-radius kernels when all radii are equal (checked once per change of sigma)."""
+        if self._unif is None:
+            self._unif = P.sigma_is_uniform(self._sigma)
+        return "uniform" if self._unif else "general"
 
     def _velocity(self, pos):
         if self.precision == "plain":
@@ -542,6 +547,7 @@ class ParticleSystem:
             L.call("vpm_core_spreading", n, sig.data_ptr(), self._nu.data_ptr(),
                    self._nut.data_ptr(), float(self.time_step_size), self._stream())
             self._sigma = sig
+            self._unif = None
         if self.relax_strength_solution:                           # :867-883
             vort = P.vorticity_field_at(self._pos, self._pos, self._str, self._sigma, self.variant)
             st2 = self._str.clone()
