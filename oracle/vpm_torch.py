@@ -63,7 +63,9 @@ def pair_sums(x, a, s, nu=None, want=("vel", "stretch", "pse")):
     for b in range(0, n, cs):
         e = min(n, b + cs)
         idx = torch.arange(b, e, device=x.device)
-        fn = lambda xi, ai, si, nui, x_, a_: _pairs(xi, ai, si, nui, idx, x_, a_, s, want)
+        # idx is bound now: the checkpoint recomputes fn during backward, after
+        # the loop has moved on
+        fn = lambda xi, ai, si, nui, x_, a_, idx=idx: _pairs(xi, ai, si, nui, idx, x_, a_, s, want)
         res = checkpoint(fn, x[b:e], a[b:e], s[b:e], nu[b:e], x, a, use_reentrant=False)
         for k, t in enumerate(res):
             outs[k].append(t)

@@ -77,11 +77,12 @@ def test_steps_follow_reference(gpu, flow, visc, method, unit, relax, layout):
 
 
 def test_fast_precisions_track_plain(gpu):
+    # DNS: a ring's LES filter is decided by roundoff (its mean vorticity is ~0)
     x, a, s, nu = FM.torus_ring(16, nu=1e-3)
     runs = {}
     for prec in ("plain", "f64", "f32c"):
         ps = ParticleSystem(time_step_size=0.02, time_integration_method="RK4",
-                            processing_unit="GPU", flow_model="LES", viscous_scheme="PSE",
+                            processing_unit="GPU", flow_model="DNS", viscous_scheme="PSE",
                             precision=prec, write_backups=False)
         ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
         for _ in range(4):
@@ -126,7 +127,8 @@ def test_particle_management_and_backup(gpu, tmp_path):
     assert np.array_equal(q.get_particle_strengths(), ps.get_particle_strengths())
     assert np.array_equal(q.get_particle_group_id(), ps.get_particle_group_id())
     total = ps.get_total_strength_magnitude()
-    ps.remove_weak_particles("relative", 1e-3, conserve_total_circulation=True)
+    # the Gaussian core's outer points are ~e^-9 of the peak; the added one 1e-9
+    ps.remove_weak_particles("relative", 1e-7, conserve_total_circulation=True)
     assert len(ps) == len(x)
     assert ps.get_total_strength_magnitude() == pytest.approx(total, rel=1e-12)
     ps.remove_particles([0, 1, 1, 10 ** 6])

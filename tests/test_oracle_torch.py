@@ -54,3 +54,24 @@ def test_chunked_autograd_equals_finite_differences():
             else:
                 fd = (loss(tx, torch.from_numpy(p)) - loss(tx, torch.from_numpy(m))) / (2 * h)
             assert float(grad[i, k]) == pytest.approx(float(fd), rel=1e-5, abs=1e-9 * float(grad.abs().max()))
+
+
+def test_multi_chunk_gradient_equals_single_chunk(monkeypatch):
+    """Chunks recomputed under checkpointing keep their own self-pair mask."""
+    x, a, s, nu = data(60, 4)
+    tx, ta = torch.from_numpy(x), torch.from_numpy(a)
+    ts, tn = torch.from_numpy(s), torch.from_numpy(nu)
+    g = torch.Generator().manual_seed(1)
+    lam = [torch.randn(60, 3, dtype=torch.float64, generator=g) for _ in range(3)]
+
+    def grads():
+        xx, aa = tx.clone().requires_grad_(True), ta.clone().requires_grad_(True)
+        u, S, D = T.pair_sums(xx, aa, ts, tn)
+        L = (lam[0] * u).sum() + (lam[1] * S).sum() + (lam[2] * D).sum()
+        return torch.autograd.grad(L, (xx, aa))
+
+    one = grads()
+    monkeypatch.setattr(T, "_chunk_size", lambda n, budget=0: 7)
+    many = grads()
+    for p, q in zip(one, many):
+        assert torch.allclose(p, q, rtol=1e-12, atol=1e-14)
