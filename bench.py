@@ -49,7 +49,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--tile-config", type=int, default=-1)
-    ap.add_argument("--cpu-targets", type=int, default=8192)
+    ap.add_argument("--cpu-targets", type=int, default=16384)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
@@ -214,6 +214,11 @@ def main(argv=None):
             "cpu_model": CB.host_cpu_model(),
         }
         log(f"[bench] CPU {pps:.3e} pairs/s ({secs:.1f} s); rel-L2 {rel}")
+        m1 = max(1, m // 64)
+        pps1, secs1, _, _ = CB.time_velocity_stretching(x, a, s, m1, nthreads=1)
+        result["cpu_baseline"]["single_core"] = {
+            "value": pps1, "sample": f"{m1} strided targets x {n} sources in {secs1:.1f} s"}
+        log(f"[bench] CPU single core {pps1:.3e} pairs/s ({secs1:.1f} s)")
 
     # ---- adjoint / forward ratio (differentiation workload size) -----------
     if not args.skip_adjoint:

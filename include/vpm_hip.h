@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VPM_ABI_VERSION 1
+#define VPM_ABI_VERSION 2
 
 #define VPM_OK 0
 #define VPM_ERR_ARG (-1)        /* bad shape, stride, option or null pointer */
@@ -149,6 +149,10 @@ typedef struct vpm_forward_opts {
   int32_t sigma_mode;   /* VPM_SIGMA_* */
   int32_t presorted;    /* 1: particles already spatially coherent, skip the Morton sort */
   int32_t tile_config;  /* -1: default; else index < vpm_tile_config_count() */
+  int32_t max_splits;   /* 0: automatic; k >= 1: at most k source splits.  When the
+                         * target blocks alone cannot fill the device, the sources
+                         * are split among workgroups and the FP64 partial sums
+                         * are added in split order (deterministic). */
 } vpm_forward_opts_t;
 
 typedef struct vpm_forward_out {

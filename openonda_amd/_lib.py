@@ -54,7 +54,7 @@ class Points(C.Structure):
 
 class ForwardOpts(C.Structure):
     _fields_ = [("outputs", C.c_uint32), ("precision", C.c_int32), ("sigma_mode", C.c_int32),
-                ("presorted", C.c_int32), ("tile_config", C.c_int32)]
+                ("presorted", C.c_int32), ("tile_config", C.c_int32), ("max_splits", C.c_int32)]
 
 
 class ForwardOut(C.Structure):
@@ -147,7 +147,7 @@ def load() -> C.CDLL:
             fn = getattr(lib, name)  # AttributeError = stale library
             fn.restype = res
             fn.argtypes = args
-        if lib.vpm_abi_version() != 1:
+        if lib.vpm_abi_version() != 2:
             raise RuntimeError("libvpm_hip.so ABI version mismatch")
         _lib = lib
         return lib

@@ -48,6 +48,15 @@ struct FusedArgs {
   double* vort;
   int64_t ld_vel, ld_str, ld_pse, ld_vort;
   double* ens_part;     // per-block enstrophy partials
+  // Source split: workgroup (b, k) sums sources [k*chunk, (k+1)*chunk) into
+  // part + k*nslot*pld (FP64 raw sums per target, by sorted index); a reduce
+  // kernel adds the splits in order k = 0, 1, ... and stores.  part == null:
+  // one split, stored directly.
+  double* part;
+  size_t part_cap;      // bytes available at part
+  int max_splits;       // 0: automatic
+  int64_t pld;
+  int chunk;
 };
 
 template <typename Real>
@@ -56,11 +65,13 @@ template <>
 struct Arith<float> {
   static __device__ inline float rsq(float a) { return __builtin_amdgcn_rsqf(a); }
   static __device__ inline float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
+  static __device__ inline float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
 template <>
 struct Arith<double> {
   static __device__ inline double rsq(double a) { return 1.0 / sqrt(a); }
   static __device__ inline double rcp(double a) { return 1.0 / a; }
+  static __device__ inline double fma(double a, double b, double c) { return ::fma(a, b, c); }
 };
 
 // LDS source record.  Uniform sigma needs position + strength; general sigma
@@ -93,11 +104,61 @@ struct Slots {
   // accumulated in FP64 per pair instead: on symmetric vortex structures the
   // term alpha_i x G is a near-exact cancellation and FP32 partials of G
   // cost an order of magnitude in the stretching's relative error.
-  static __device__ constexpr bool used(int k) {
+  static __host__ __device__ constexpr bool used(int k) {
     return (k < 3 && vel) || (k >= 6 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
            (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
   }
+  // position of slot k among the used ones (the split partial layout)
+  static __host__ __device__ constexpr int packed(int k) {
+    int c = 0;
+    for (int q = 0; q < k; ++q) c += used(q) ? 1 : 0;
+    return c;
+  }
+  static constexpr int nacc = packed(kNAcc);
+  static constexpr int nslot = nacc + (str ? 3 : 0);  // + the FP64 alpha_j sum
 };
+
+// Scaling and storing of one target's totals T (FP32-pass slots, FP64) and G
+// (the FP64 alpha_j sum of the stretching); returns its enstrophy term.
+template <unsigned OUT, bool UNIF>
+__device__ __forceinline__ double store_target(const FusedArgs& a, const Src8& ti, int64_t o,
+                                               const double* T, const double* G, double scale) {
+  using S = Slots<OUT>;
+  const double C = kOneOverFourPi;
+  double s1 = 1.0, s2 = 1.0, s3 = 1.0;  // 1/sigma^k for uniform sigma
+  if constexpr (UNIF) {
+    s1 = scale;
+    s2 = scale * scale;
+    s3 = s2 * scale;
+  }
+  if constexpr (S::vel) {
+    const double f = -C * s2;
+    a.vel[o] = f * T[kU + 0];
+    a.vel[a.ld_vel + o] = f * T[kU + 1];
+    a.vel[2 * a.ld_vel + o] = f * T[kU + 2];
+  }
+  if constexpr (S::str) {
+    const double ax = ti.ax, ay = ti.ay, az = ti.az;
+    const double f = C * s3;
+    a.str[o] = f * ((ay * G[2] - az * G[1]) + T[kSd + 0]);
+    a.str[a.ld_str + o] = f * ((az * G[0] - ax * G[2]) + T[kSd + 1]);
+    a.str[2 * a.ld_str + o] = f * ((ax * G[1] - ay * G[0]) + T[kSd + 2]);
+  }
+  if constexpr (S::pse) {
+    const double f = 105.0 * ti.nu * C * s2;
+    a.pse[o] = f * T[kP + 0];
+    a.pse[a.ld_pse + o] = f * T[kP + 1];
+    a.pse[2 * a.ld_pse + o] = f * T[kP + 2];
+  }
+  if constexpr (S::vort) {
+    const double f = 7.5 * C * s3;
+    a.vort[o] = f * T[kW + 0];
+    a.vort[a.ld_vort + o] = f * T[kW + 1];
+    a.vort[2 * a.ld_vort + o] = f * T[kW + 2];
+  }
+  if constexpr (S::ens) return (ti.ax * T[kE + 0] + ti.ay * T[kE + 1] + ti.az * T[kE + 2]) * s1;
+  return 0.0;
+}
 
 template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
@@ -108,8 +169,8 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
   // A = rho^2 + 1.  With a uniform radius (offsets pre-scaled by 1/sigma) and
   // no vorticity mask to evaluate, the +1 starts the FMA chain; every factor
-  // below is written in A: rho^2 + 2.5 = A + 1.5, 3 rho^2 + 10.5 = 3A + 7.5,
-  // rho^2 + 3.2 = A + 2.2.
+  // below is written in A (rho^2 + 2.5 = A + 1.5, 3 rho^2 + 10.5 = 3A + 7.5,
+  // rho^2 + 3.2 = A + 2.2) and then in powers of t = A^-1/2.
   Real r2 = Real(0), A, inv = Real(1), inv2 = Real(1);
   if constexpr (UNIF && !S::vort) {
     if constexpr (std::is_same<Real, float>::value)
@@ -127,13 +188,15 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
       A = r2 * inv2 + Real(1);
     }
   }
+  // Since A t^2 = 1:  (A + 1.5) t^5 = t^3 + 1.5 t^5,  (3A + 7.5) t^7 = t^5 (3 + 7.5 t^2)
+  // and (A + 2.2) t^3 = t + 2.2 t^3 -- one instruction fewer each, and fewer
+  // factors of the (1-ulp) hardware rsqrt than the A-weighted forms.
   const Real t1 = M::rsq(A);
   const Real t2 = t1 * t1;
   const Real t3 = t2 * t1;
   const Real t5 = t3 * t2;
-  const Real t7 = t5 * t2;
   if constexpr (S::vel || S::str) {
-    Real g = (A + Real(1.5)) * t5;
+    Real g = M::fma(Real(1.5), t5, t3);
     if constexpr (!UNIF) g *= inv2 * inv;
     const Real cx = dy * s.az - dz * s.ay;
     const Real cy = dz * s.ax - dx * s.az;
@@ -149,7 +212,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
       g64[1] = fma(gd, aj64[1], g64[1]);
       g64[2] = fma(gd, aj64[2], g64[2]);
       const Real e = t.ax * cx + t.ay * cy + t.az * cz;
-      Real f2 = (Real(3) * A + Real(7.5)) * t7;
+      Real f2 = t5 * M::fma(Real(7.5), t2, Real(3));
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
       const Real w = f2 * e;
       acc[kSd + 0] += w * dx;
@@ -158,7 +221,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
     }
   }
   if constexpr (S::pse) {
-    Real t9 = t7 * t2;
+    Real t9 = t5 * t2 * t2;
     if constexpr (!UNIF) t9 *= inv2;
     acc[kP + 0] += t9 * (s.ax - t.ax);
     acc[kP + 1] += t9 * (s.ay - t.ay);
@@ -167,7 +230,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   if constexpr (S::vort) {
     Real wv;
     if constexpr (UNIF) {
-      wv = t7;
+      wv = t5 * t2;
     } else {  // sigma_j, not sigma_ij (ParticlesPhysicsCuda.py:323,334-337)
       const Real Aj = r2 * s.is2 + Real(1);
       const Real tj = M::rsq(Aj);
@@ -180,7 +243,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
     acc[kW + 2] += wv * s.az;
   }
   if constexpr (S::ens) {
-    Real en = (A + Real(2.2)) * t3;
+    Real en = M::fma(Real(2.2), t3, t1);
     if constexpr (!UNIF) en *= inv;
     acc[kE + 0] += en * s.ax;
     acc[kE + 1] += en * s.ay;
@@ -189,11 +252,15 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
 }
 
 // One workgroup's work: BLOCK*R consecutive sorted targets of a particle set
-// of n particles (src, perm) starting at `base`, against all n sources.
-// Writes the outputs of its targets and its enstrophy partial to *ens_slot.
+// of n particles (src, perm) starting at `base`, against sources [lo, hi).
+// split_out == null: writes the outputs of its targets and its enstrophy
+// partial to *ens_slot.  Otherwise writes its raw sums to split_out (Slots::packed layout,
+// leading dimension pld) for the split reduce.
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
 __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, const Src8* src,
-                                            const int* perm, const int base, double* ens_slot) {
+                                            const int* perm, const int base, double* ens_slot,
+                                            const int lo, const int hi, double* split_out,
+                                            const int64_t pld) {
   static_assert(TILE % SUB == 0 && TILE % BLOCK == 0, "tile shape");
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
@@ -282,13 +349,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   for (int r = 0; r < R; ++r) gsum[r][0] = gsum[r][1] = gsum[r][2] = 0.0;
 
   // ---- sweep the source tiles ------------------------------------------
-  for (int t0 = 0; t0 < n; t0 += TILE) {
+  for (int t0 = lo; t0 < hi; t0 += TILE) {
     __syncthreads();
 #pragma unroll
     for (int k = tid; k < TILE; k += BLOCK) {
       const int j = t0 + k;
       SrcR<Real, UNIF> s;
-      if (j < n) {
+      if (j < hi) {
         const Src8 g = src[j];
         s.x = static_cast<Real>((g.x - cx) * scale);
         s.y = static_cast<Real>((g.y - cy) * scale);
@@ -376,50 +443,27 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   }
 
   // ---- epilogue ----------------------------------------------------------
-  const double C = kOneOverFourPi;
-  double s1 = 1.0, s2 = 1.0, s3 = 1.0;  // 1/sigma^k for uniform sigma
-  if constexpr (UNIF) {
-    s1 = scale;
-    s2 = scale * scale;
-    s3 = s2 * scale;
+  if (split_out) {  // one split of several: raw sums, coalesced by sorted index
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!valid[r]) continue;
+      const int k = base + r * BLOCK + tid;
+#pragma unroll
+      for (int q = 0; q < kNAcc; ++q)
+        if (S::used(q)) split_out[S::packed(q) * pld + k] = tot[r][q];
+      if constexpr (S::str) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) split_out[(S::nacc + c) * pld + k] = gsum[r][c];
+      }
+    }
+    return;
   }
   double ens_local = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (!valid[r]) continue;
     const int k = base + r * BLOCK + tid;
-    const int64_t o = perm ? perm[k] : k;
-    const double* T = tot[r];
-    const Src8 ti = src[k];
-    if constexpr (S::vel) {
-      const double f = -C * s2;
-      a.vel[o] = f * T[kU + 0];
-      a.vel[a.ld_vel + o] = f * T[kU + 1];
-      a.vel[2 * a.ld_vel + o] = f * T[kU + 2];
-    }
-    if constexpr (S::str) {
-      const double ax = ti.ax, ay = ti.ay, az = ti.az;
-      const double gx = gsum[r][0], gy = gsum[r][1], gz = gsum[r][2];
-      const double f = C * s3;
-      a.str[o] = f * ((ay * gz - az * gy) + T[kSd + 0]);
-      a.str[a.ld_str + o] = f * ((az * gx - ax * gz) + T[kSd + 1]);
-      a.str[2 * a.ld_str + o] = f * ((ax * gy - ay * gx) + T[kSd + 2]);
-    }
-    if constexpr (S::pse) {
-      const double f = 105.0 * ti.nu * C * s2;
-      a.pse[o] = f * T[kP + 0];
-      a.pse[a.ld_pse + o] = f * T[kP + 1];
-      a.pse[2 * a.ld_pse + o] = f * T[kP + 2];
-    }
-    if constexpr (S::vort) {
-      const double f = 7.5 * C * s3;
-      a.vort[o] = f * T[kW + 0];
-      a.vort[a.ld_vort + o] = f * T[kW + 1];
-      a.vort[2 * a.ld_vort + o] = f * T[kW + 2];
-    }
-    if constexpr (S::ens) {
-      ens_local += (ti.ax * T[kE + 0] + ti.ay * T[kE + 1] + ti.az * T[kE + 2]) * s1;
-    }
+    ens_local += store_target<OUT, UNIF>(a, src[k], perm ? perm[k] : k, tot[r], gsum[r], scale);
   }
   if constexpr (S::ens) {
     __syncthreads();
@@ -428,10 +472,47 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   }
 }
 
+// grid (target blocks, splits)
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
 __global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
+  const int k = static_cast<int>(blockIdx.y);
+  const int lo = k * a.chunk;
+  const int hi = a.chunk < a.n - lo ? lo + a.chunk : a.n;
+  double* part = a.part ? a.part + static_cast<int64_t>(k) * Slots<OUT>::nslot * a.pld : nullptr;
   fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
-      a, a.n, a.src, a.perm, static_cast<int>(blockIdx.x) * (BLOCK * R), a.ens_part + blockIdx.x);
+      a, a.n, a.src, a.perm, static_cast<int>(blockIdx.x) * (BLOCK * R), a.ens_part + blockIdx.x,
+      lo, hi, part, a.pld);
+}
+
+// Adds the splits' raw sums in split order (deterministic) and stores; one
+// enstrophy partial per workgroup.
+template <unsigned OUT, bool UNIF>
+__global__ __launch_bounds__(256) void split_reduce_kernel(FusedArgs a, int nsplit) {
+  using S = Slots<OUT>;
+  __shared__ double red[4];
+  const int k = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  double ens_local = 0.0;
+  if (k < a.n) {
+    double T[kNAcc], G[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kNAcc; ++q) T[q] = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) {
+      const double* P = a.part + static_cast<int64_t>(sp) * S::nslot * a.pld;
+#pragma unroll
+      for (int q = 0; q < kNAcc; ++q)
+        if (S::used(q)) T[q] += P[S::packed(q) * a.pld + k];
+      if constexpr (S::str) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G[c] += P[(S::nacc + c) * a.pld + k];
+      }
+    }
+    const double scale = UNIF ? 1.0 / a.src[0].s : 1.0;
+    ens_local = store_target<OUT, UNIF>(a, a.src[k], a.perm ? a.perm[k] : k, T, G, scale);
+  }
+  if constexpr (S::ens) {
+    const double bs = block_sum<256>(ens_local, red);
+    if (threadIdx.x == 0) a.ens_part[blockIdx.x] = bs;
+  }
 }
 
 // ------------------------------------------------------------------ configs
@@ -448,14 +529,86 @@ constexpr TileCfg kCfgs[] = {
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
 
+// Most workgroups in flight at once on the current device for `kern`.
+template <typename Kern>
+int resident_blocks(Kern kern, int block) {
+  static const int per_cu = [&] {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0) != hipSuccess) occ = 1;
+    return occ > 0 ? occ : 1;
+  }();
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 1;
+  return per_cu * cus;
+}
+
+// Splits actually used when `k` are asked for: whole tiles per split, none empty.
+int effective_splits(int ntiles, int k) {
+  const int per = (ntiles + k - 1) / k;
+  return (ntiles + per - 1) / per;
+}
+
+// Number of source splits for `blocks` target blocks: equal-work workgroups
+// run in rounds of `resident`, so take the smallest split count (<= kmax)
+// whose launch fills its rounds best -- if that beats no split by 5%.
+int choose_splits(int blocks, int ntiles, int kmax, int resident) {
+  auto fill = [&](int k) {
+    const int64_t b = static_cast<int64_t>(blocks) * effective_splits(ntiles, k);
+    const int64_t rounds = (b + resident - 1) / resident;
+    return static_cast<double>(b) / static_cast<double>(rounds * resident);
+  };
+  int best = 1;
+  double best_fill = fill(1);
+  for (int k = 2; k <= kmax; ++k) {
+    const double f = fill(k);
+    if (f > best_fill) {
+      best = k;
+      best_fill = f;
+    }
+  }
+  return best_fill >= fill(1) + 0.05 ? effective_splits(ntiles, best) : 1;
+}
+
+constexpr int kMaxSplitBlocks = 4096;
+
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-int launch_cfg(const FusedArgs& a, hipStream_t st, int* nblocks) {
-  const int blocks = static_cast<int>(ceil_div(a.n, BLOCK * R));
+int launch_cfg(const FusedArgs& a0, hipStream_t st, int* nblocks) {
+  const int blocks = static_cast<int>(ceil_div(a0.n, BLOCK * R));
   *nblocks = blocks;
   if (blocks == 0) return VPM_OK;
-  hipLaunchKernelGGL((fused_kernel<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>), dim3(blocks),
-                     dim3(BLOCK), 0, st, a);
+  auto kern = fused_kernel<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>;
+  FusedArgs a = a0;
+  const int ntiles = static_cast<int>(ceil_div(a.n, TILE));
+  const int64_t pld = static_cast<int64_t>(blocks) * BLOCK * R;
+  const size_t per_split = static_cast<size_t>(Slots<OUT>::nslot) * pld * sizeof(double);
+  int ks = 1;
+  if (a.part && blocks < kMaxSplitBlocks / 2) {
+    const size_t by_cap = a.part_cap / per_split;
+    int kmax = std::min(ntiles, kMaxSplitBlocks / blocks);
+    if (by_cap < static_cast<size_t>(kmax)) kmax = static_cast<int>(by_cap);
+    if (a.max_splits > 0 && a.max_splits < kmax) kmax = a.max_splits;
+    if (kmax > 1) ks = choose_splits(blocks, ntiles, kmax, resident_blocks(kern, BLOCK));
+  }
+  if (ks > 1) {
+    const int per = (ntiles + ks - 1) / ks;  // tiles per split; ks is effective
+    a.chunk = per * TILE;
+    a.pld = pld;
+  } else {
+    a.part = nullptr;
+    a.chunk = a.n;
+    a.pld = 0;
+  }
+  hipLaunchKernelGGL(kern, dim3(blocks, ks), dim3(BLOCK), 0, st, a);
   VPM_LAUNCH_CHECK("fused_kernel");
+  if (ks > 1) {
+    const int rb = static_cast<int>(ceil_div(a.n, 256));
+    hipLaunchKernelGGL((split_reduce_kernel<OUT, UNIF>), dim3(rb), dim3(256), 0, st, a, ks);
+    VPM_LAUNCH_CHECK("split_reduce_kernel");
+    *nblocks = rb;
+  }
   return VPM_OK;
 }
 
@@ -521,10 +674,34 @@ int dispatch(const FusedArgs& a, unsigned mask, int prec, bool unif, int cfg, hi
 
 // ------------------------------------------------------------------ workspace
 struct WsLayout {
-  size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, scratch_vec, cub,
-      total;
-  size_t cub_bytes;
+  size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, scratch_vec, split,
+      cub, total;
+  size_t cub_bytes, split_bytes;
 };
+
+// FP64 partial slots per target of a source split (see Slots::nslot)
+int split_slots(unsigned mask) {
+  int c = 0;
+  for (unsigned b = 1; b <= VPM_OUT_ENSTROPHY; b <<= 1) c += (mask & b) ? 3 : 0;
+  return c + ((mask & VPM_OUT_STRETCH) ? 3 : 0);
+}
+
+// Room for the largest source split launch_cfg may choose for this call.
+size_t split_capacity(int64_t n, const vpm_forward_opts_t* o) {
+  if (!o || n <= 0) return 0;
+  int br = kCfgF64.block * kCfgF64.r, tile = kCfgF64.tile;
+  if (o->precision == VPM_PREC_F32C) {
+    const int c = (o->tile_config < 0 || o->tile_config >= kNumCfgs) ? 0 : o->tile_config;
+    br = kCfgs[c].block * kCfgs[c].r;
+    tile = kCfgs[c].tile;
+  }
+  const int64_t blocks = ceil_div(n, br);
+  if (blocks >= kMaxSplitBlocks / 2) return 0;
+  const int64_t kmax = std::min<int64_t>(ceil_div(n, tile), kMaxSplitBlocks / blocks);
+  if (kmax < 2) return 0;
+  return static_cast<size_t>(kmax) * split_slots(kernel_mask(o->outputs)) *
+         static_cast<size_t>(blocks * br) * sizeof(double);
+}
 
 // The hipCUB scratch comes last, so the other offsets do not depend on it;
 // calls that do not sort pass need_cub = false and skip the size query.
@@ -546,6 +723,8 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = 
   L->ens_part = take((N / 64 + 2) * sizeof(double));
   // Unrequested outputs of a superset kernel land here: 3 vectors of N.
   L->scratch_vec = take(3 * N * sizeof(double));
+  L->split_bytes = split_capacity(n, o);
+  L->split = take(L->split_bytes);
   size_t cub_bytes = 0;
   if (need_cub && (!o || !o->presorted)) {
     const int rc = sort_temp_bytes(n, &cub_bytes);
@@ -589,7 +768,8 @@ __global__ __launch_bounds__(BLOCK) void fused_batch_kernel(BatchArgs b) {
     const int off = b.inst_off[inst];
     const int n = b.inst_off[inst + 1] - off;
     fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
-        b.f, n, b.src + off, b.perm + off, b.item_block[item] * (BLOCK * R), b.f.ens_part + item);
+        b.f, n, b.src + off, b.perm + off, b.item_block[item] * (BLOCK * R), b.f.ens_part + item,
+        0, n, nullptr, 0);
   }
 }
 
@@ -888,6 +1068,10 @@ int check_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, const c
     set_error("%s: tile_config out of range", who);
     return VPM_ERR_ARG;
   }
+  if (o->max_splits < 0) {
+    set_error("%s: max_splits < 0", who);
+    return VPM_ERR_ARG;
+  }
   return check_particles(p, (want & VPM_OUT_PSE) != 0, who);
 }
 
@@ -979,6 +1163,9 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   route(VPM_OUT_PSE, out->pse, &a.pse, &a.ld_pse);
   route(VPM_OUT_VORT, out->vort, &a.vort, &a.ld_vort);
   a.ens_part = reinterpret_cast<double*>(ws + L.ens_part);
+  a.part = L.split_bytes ? reinterpret_cast<double*>(ws + L.split) : nullptr;
+  a.part_cap = L.split_bytes;
+  a.max_splits = o->max_splits;
   int nblocks = 0;
   const int cfg = o->tile_config < 0 ? 0 : o->tile_config;
   if ((rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st,

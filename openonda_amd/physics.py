@@ -196,7 +196,7 @@ def sigma_is_uniform(sigma: torch.Tensor) -> bool:
 
 
 def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f32c",
-            sigma_mode="auto", presorted=False, tile_config=-1, out=None):
+            sigma_mode="auto", presorted=False, tile_config=-1, max_splits=0, out=None):
     """One fused all-pairs pass (GPU-plugin semantics) returning a dict.
 
     ``outputs`` is any subset of {"vel", "stretch", "pse", "vort",
@@ -206,6 +206,9 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
     checked), "general" uses sigma_ij per pair, "auto" checks on the device.
     ``tile_config``: -1 (default shape), an index of :func:`tile_configs`, or
     "auto" (the autotuner's cached winner for this N).
+    ``max_splits``: 0 lets the library split the sources among workgroups
+    when the targets alone cannot fill the device (small N); k >= 1 caps
+    the number of splits (1: never split).
     ``out`` may supply preallocated tensors by name.
     """
     if isinstance(outputs, str):
@@ -234,7 +237,7 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
             if precision == "f32c" else -1
     opts = L.ForwardOpts(mask, PRECISIONS[precision],
                          L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL,
-                         1 if presorted else 0, int(tile_config))
+                         1 if presorted else 0, int(tile_config), int(max_splits))
     out = dict(out or {})
     res = {}
     ld = None

@@ -30,6 +30,8 @@ def load():
         lib.vpm_cpu_strength_gradients_pse.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int64, _L, _D]
         lib.vpm_cpu_strength_gradients_pse.restype = None
         lib.vpm_cpu_max_threads.restype = C.c_int
+        lib.vpm_cpu_set_threads.argtypes = [C.c_int]
+        lib.vpm_cpu_set_threads.restype = None
         _lib = lib
     return _lib
 
@@ -74,17 +76,24 @@ def threads() -> int:
     return int(load().vpm_cpu_max_threads())
 
 
-def time_velocity_stretching(x, a, s, m):
+def time_velocity_stretching(x, a, s, m, nthreads=None):
     """Times velocity + stretching (two passes, as the reference runs them) for
-    m strided targets against all sources.  Returns (pairs_per_s, seconds,
-    (u, S), tidx)."""
+    m strided targets against all sources on `nthreads` threads (default: all).
+    Returns (pairs_per_s, seconds, (u, S), tidx)."""
     n = len(x)
     step = max(1, n // m)
     tidx = np.arange(0, n, step, dtype=np.int64)[:m]
-    t0 = time.perf_counter()
-    u = selfinduced_velocity(x, a, s, tidx)
-    S = strength_gradients(x, a, s, tidx)
-    dt = time.perf_counter() - t0
+    lib = load()
+    full = threads()
+    if nthreads:
+        lib.vpm_cpu_set_threads(int(nthreads))
+    try:
+        t0 = time.perf_counter()
+        u = selfinduced_velocity(x, a, s, tidx)
+        S = strength_gradients(x, a, s, tidx)
+        dt = time.perf_counter() - t0
+    finally:
+        lib.vpm_cpu_set_threads(full)
     return len(tidx) * n / dt, dt, (u, S), tidx
 
 

@@ -131,3 +131,13 @@ int vpm_cpu_max_threads(void) {
   return 1;
 #endif
 }
+
+/* threads used by the following parallel loops of the calling thread */
+void vpm_cpu_set_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}

@@ -169,3 +169,36 @@ def test_autotune_records_a_valid_winner(gpu, tmp_path):
     assert all(r["rel_l2"] <= F32C_TOL for r in report)
     assert AT.best_config(1 << 14, ("vel", "stretch"), path=path) == best["config"]
     assert AT.best_config(1 << 20, ("vel", "stretch"), path=path) == -1
+
+
+SPLIT_CASES = [
+    # n, precision, sigma_mode, outputs
+    (8192, "f32c", "uniform", ("vel", "stretch")),
+    (8192, "f32c", "uniform", ("vel", "stretch", "pse", "vort", "enstrophy")),
+    (4096, "f32c", "general", ("vel", "stretch", "pse", "vort", "enstrophy")),
+    (4096, "f64", "uniform", ("vel", "stretch", "pse")),
+    (3000, "f64", "general", ("vel", "stretch", "pse", "vort", "enstrophy")),
+]
+
+
+@pytest.mark.parametrize("n,precision,mode,outputs", SPLIT_CASES)
+def test_source_split_matches_single_pass(gpu, n, precision, mode, outputs):
+    """Sources split among workgroups (small N) give the unsplit result."""
+    print(f"[split] n={n} {precision} {mode} {outputs}", flush=True)
+    x, a, s, nu = FM.random_cloud(n)
+    nu = np.full(n, 1e-3)
+    if mode == "general":
+        s = s * (1.0 + 0.2 * np.random.default_rng(n).random(n))
+    d = to_dev(x, a, s, nu)
+    auto = P.forward(*d, outputs=outputs, precision=precision, sigma_mode=mode)
+    single = P.forward(*d, outputs=outputs, precision=precision, sigma_mode=mode, max_splits=1)
+    two = P.forward(*d, outputs=outputs, precision=precision, sigma_mode=mode, max_splits=2)
+    tol = 1e-13 if precision == "f64" else 2e-7
+    for k in outputs:
+        ref = single[k].cpu().numpy()
+        for name, res in (("auto", auto), ("two", two)):
+            got = res[k].cpu().numpy()
+            err = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300)
+            assert err <= tol, (name, k, err)
+    ref = oracle_all(x, a, s, nu)
+    check({k: auto[k] for k in outputs}, ref, F64_TOL if precision == "f64" else F32C_TOL, "auto")
