@@ -19,6 +19,8 @@ import math
 
 import numpy as np
 
+from .vpm_flow_models import lamb_oseen_vpm, vortex_ring_vpm  # noqa: F401  (re-exported)
+
 CROSS_SECTION_COUNTS = (1, 10, 21, 32)
 
 
@@ -120,53 +122,6 @@ def batch_instances(count=96, nmin=2048, nmax=131072, seed=5):
         else:
             out.append(random_cloud(int(n), seed=1000 + i))
     return out
-
-
-# ------------------------------------------------------------------ reference generators
-def vortex_ring_vpm(particle_positions, volumes, particle_radius, particle_viscosity,
-                    ring_center, ring_radius, ring_strength, ring_thickness,
-                    epsilon_W=0.0, phase_W=0.0, N_W=12):
-    """Gaussian-core vortex ring with an azimuthal core perturbation.
-
-    Restates vpm_flow_models.py:86-159 (arguments and returns identical):
-    returns (velocities, strengths, radii, viscosities).
-    """
-    p = np.asarray(particle_positions, dtype=np.float64)
-    vol = np.asarray(volumes, dtype=np.float64)
-    X = p[:, 0] - ring_center[0]
-    Y = p[:, 1] - ring_center[1]
-    Z = p[:, 2] - ring_center[2]
-    theta = np.arctan2(Z, Y)                                       # :127
-    Y = Y - epsilon_W * np.cos(phase_W + N_W * theta)              # :128-131
-    Z = Z - epsilon_W * np.sin(phase_W + N_W * theta)
-    r_yz = np.sqrt(Y ** 2 + Z ** 2)                                # :134
-    d = np.abs(r_yz - ring_radius)
-    w = (ring_strength / (np.pi * ring_thickness ** 2)) * np.exp(-d ** 2 / ring_thickness ** 2)
-    xd = np.exp(-X ** 2 / ring_thickness ** 2)                     # :137
-    s = np.empty_like(p)
-    s[:, 0] = 0.0                                                  # :140-148
-    s[:, 1] = -w * np.sin(theta) * xd * vol
-    s[:, 2] = w * np.cos(theta) * xd * vol
-    g = np.sum(np.sqrt(np.sum(s ** 2, axis=1)))                    # :150-152
-    s *= np.abs(2 * np.pi * ring_radius * ring_strength / g)
-    n = len(p)
-    return (np.zeros_like(s), s, np.full(n, particle_radius, dtype=np.float64),
-            np.full(n, particle_viscosity, dtype=np.float64))
-
-
-def lamb_oseen_vpm(particle_positions, volumes, particle_radius, particle_viscosity,
-                   vortex_center, vortex_strength, core_radius, epsilon_W=0.0, phase_W=0.0,
-                   N_W=12):
-    """Perturbed Lamb-Oseen column (vpm_flow_models.py:13-82)."""
-    p = np.asarray(particle_positions, dtype=np.float64)
-    n = len(p)
-    px = vortex_center[0] + epsilon_W * np.sin(phase_W + N_W * p[:, 2])
-    py = vortex_center[1] + epsilon_W * np.cos(phase_W + N_W * p[:, 2])
-    r = np.sqrt((p[:, 0] - px) ** 2 + (p[:, 1] - py) ** 2)
-    s = np.zeros_like(p)
-    s[:, 2] = (vortex_strength / (np.pi * core_radius ** 2)) * np.exp(-r ** 2 / core_radius ** 2) \
-        * np.asarray(volumes, dtype=np.float64)
-    return (np.zeros_like(s), s, particle_radius * np.ones(n), particle_viscosity * np.ones(n))
 
 
 def rectangular_point_distribution(box_limits, spacing):

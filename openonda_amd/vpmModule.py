@@ -452,6 +452,46 @@ class ParticleSystem:
             "Viscosity_t": self.get_particle_viscosities_t()})
         self._log(f"• Particle data exported to {name}")
 
+    # The VTP backup above holds the particle fields only, as the reference's
+    # does.  A checkpoint adds what a bit-identical resume also needs: the
+    # strength history of the multistep update (vpmModule.py:741-746,783-787),
+    # the LES filter state (C_nu, enstrophy reference; :887-909) and the clock.
+    _CKPT_VERSION = 1
+
+    def save_checkpoint(self, filename: str):
+        """Writes the full solver state to an .npz file (plain arrays, no pickles)."""
+        arrays = {
+            "version": np.array(self._CKPT_VERSION, dtype=np.int64),
+            "positions": self._pos.cpu().numpy(), "velocities": self._vel.cpu().numpy(),
+            "strengths": self._str.cpu().numpy(), "radii": self._sigma.cpu().numpy(),
+            "viscosities": self._nu.cpu().numpy(), "viscosities_t": self._nut.cpu().numpy(),
+            "group_id": self._gid.cpu().numpy(), "les": self._les.cpu().numpy(),
+            "clock": np.array([self.flow_time, float(self.time_step), self.time_step_size]),
+            "has_history": np.array(self._hist is not None),
+        }
+        if self._hist is not None:
+            arrays["history"] = torch.stack(self._hist).cpu().numpy()
+        with open(filename, "wb") as f:
+            np.savez(f, **arrays)
+        self._log(f"• Checkpoint written to {filename}")
+
+    def load_checkpoint(self, filename: str):
+        """Restores the state written by :meth:`save_checkpoint` (replacing the
+        current particles); the solver settings of this instance are kept."""
+        with np.load(filename, allow_pickle=False) as z:
+            if int(z["version"]) != self._CKPT_VERSION:
+                raise ValueError(f"unsupported checkpoint version {int(z['version'])}")
+            d, f64 = self.device, torch.float64
+            t = lambda k, dt=f64: torch.from_numpy(np.ascontiguousarray(z[k])).to(d, dt)
+            self._set_state(t("positions"), t("velocities"), t("strengths"), t("radii"),
+                            t("viscosities"), t("viscosities_t"), t("group_id", torch.int64))
+            self._les = t("les")
+            self.flow_time, step, self.time_step_size = (float(v) for v in z["clock"])
+            self.time_step = int(step)
+            if bool(z["has_history"]):
+                h = t("history")
+                self._hist = [h[k].contiguous() for k in range(h.shape[0])]
+
     # ------------------------------------------------------------ time integration
     def update_state(self):                                        # :522-560
         """Advances positions with the chosen RK method (stages use the

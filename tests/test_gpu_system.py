@@ -150,3 +150,34 @@ def test_constructor_surface(gpu):
         ParticleSystem(time_integration_method="RK5")
     with pytest.raises(TypeError):
         ParticleSystem(time_step_size="1")
+
+
+@pytest.mark.parametrize("flow,visc", [("LES", "PSE"), ("DNS", "CoreSpreading")])
+def test_checkpoint_resume_is_bit_identical(gpu, tmp_path, flow, visc):
+    """Stepping on from a checkpoint reproduces the uninterrupted run exactly."""
+    x, a, s, nu = cloud_with_mean(300, 9)
+
+    def system():
+        return ParticleSystem(time_step_size=0.02, time_integration_method="RK3",
+                              processing_unit="GPU", flow_model=flow, viscous_scheme=visc,
+                              write_backups=False)
+
+    def step(ps, k):
+        for _ in range(k):
+            ps.update_velocities()
+            ps.update_strengths()
+            ps.update_state()
+
+    ps = system()
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu, group_id=2)
+    step(ps, 3)
+    ps.save_checkpoint(str(tmp_path / "c.npz"))
+    step(ps, 2)
+    q = system()
+    q.load_checkpoint(str(tmp_path / "c.npz"))
+    assert q.time_step == 3 and q.flow_time == pytest.approx(0.06)
+    step(q, 2)
+    for get in ("get_particle_positions", "get_particle_strengths", "get_particle_radii",
+                "get_particle_velocities", "get_particle_viscosities_t", "get_particle_group_id"):
+        assert np.array_equal(getattr(q, get)(), getattr(ps, get)()), get
+    assert q.Cnu == ps.Cnu and q.time_step == ps.time_step
