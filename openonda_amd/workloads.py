@@ -15,6 +15,11 @@ workload:
 * ``rollout``   -- ring N = 131,072, 50 RK4 steps (dt = 0.005), probe-velocity
                    loss against a ring of radius 1.05, gradient by the
                    hand-written adjoint with checkpoints every 10 steps.
+* ``examples``  -- the reference's example cases E1-E5 at their scripts'
+                   resolution through ParticleSystem (plain FP64, as
+                   scripted), ``--steps`` driver steps each (default 10):
+                   particles, ms per step, circulation and kinetic-energy
+                   drift.
 
 The FP64 references on the GPU are the library's FP64 kernels, which the
 parity tests hold to the CPU oracle.
@@ -173,9 +178,39 @@ def rollout(dev, n_st=2048, steps=50, dt=0.005, ck=10, probes=4096):
             "peak_mem_GB": torch.cuda.max_memory_allocated(dev) / 1e9}
 
 
+def examples(dev, steps=10, names=("E1", "E2", "E3", "E4", "E5"), precision="plain"):
+    from . import examples as EX
+    from .vpmModule import ParticleSystem
+
+    rows = []
+    for name in names:
+        case = EX.CASES[name]()
+        ps = ParticleSystem(**case.system, precision=precision, write_backups=False)
+        for p, v, a, s, nu, gid in case.groups:
+            ps.add_particle_field(p, v, a, s, nu, group_id=gid)
+        ps.remove_weak_particles(**case.remove_weak)
+        c0, e0 = ps.get_total_strength_magnitude(), ps.get_total_kinetic_energy()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ps.update_velocities()
+            ps.update_strengths()
+            ps.update_state()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        row = {"case": name, "n": len(ps), "steps": steps, "of": case.n_time_steps,
+               "flow_model": case.system["flow_model"], "ms_per_step": ms,
+               "circulation_drift": ps.get_total_strength_magnitude() / c0 - 1.0,
+               "kinetic_energy_drift": ps.get_total_kinetic_energy() / e0 - 1.0}
+        _log(f"[examples] {row}")
+        rows.append(row)
+    return {"workload": f"reference examples E1-E5 at script resolution, {steps} steps each",
+            "rows": rows}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("name", choices=["ring", "leapfrog", "sweep", "rollout"])
+    ap.add_argument("name", choices=["ring", "leapfrog", "sweep", "rollout", "examples"])
     ap.add_argument("--steps", type=int, default=None)
     a = ap.parse_args(argv)
     P.require_gpu()
@@ -186,6 +221,8 @@ def main(argv=None):
         r = leapfrog(dev, steps=a.steps or 200)
     elif a.name == "sweep":
         r = sweep(dev)
+    elif a.name == "examples":
+        r = examples(dev, steps=a.steps or 10)
     else:
         r = rollout(dev, steps=a.steps or 50)
     print(json.dumps(r), flush=True)
