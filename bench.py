@@ -38,6 +38,11 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 FLOPS_PER_PAIR_VEL_STRETCH = 53
 
 
+def traffic_workload(n):
+    """Name of the workload in the traffic summaries under profiles/."""
+    return f"random cloud N={n}, velocity+stretching, f32c"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -133,6 +138,14 @@ def main(argv=None):
     log(f"[bench] f32c: {ms_step:.2f} ms/step, kernel {kern_ms:.2f} ms, {value:.3e} pairs/s, "
         f"{achieved:.1f} TFLOP/s")
 
+    # HBM bytes per launch of this kernel instance on this workload, from the
+    # calibrated FETCH_SIZE + WRITE_SIZE runs summarised under profiles/
+    from openonda_amd import profiling
+
+    cfg = P.tile_configs()[args.tile_config if args.tile_config >= 0 else 0]
+    kernel_name = "fused_kernel<float, 3u, true, {}, {}, {}, {}, {}>".format(*cfg)
+    traffic = profiling.load_traffic(traffic_workload(n), kernel_name)
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -159,8 +172,8 @@ def main(argv=None):
             "peak": FP32_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "fused_kernel<float, VEL|STRETCH, uniform sigma>",
+            "traffic": traffic,
+            "kernel": kernel_name,
             "kernel_ms": kern_ms,
             "flops_per_pair": FLOPS_PER_PAIR_VEL_STRETCH,
         },

@@ -160,10 +160,18 @@ __device__ __forceinline__ double store_target(const FusedArgs& a, const Src8& t
   return 0.0;
 }
 
+// Non-inline constants of the pair update, held in registers by the caller:
+// as literal operands (v_fmac_f32 v, 0x3fc00000, v) they halve the issue rate
+// of their instruction on gfx950.
+template <typename Real>
+struct PairConst {
+  Real c15, c75, c22, c3;
+};
+
 template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
                                             const SrcR<Real, UNIF>& s, const double* aj64,
-                                            Real eps_r2) {
+                                            Real eps_r2, const PairConst<Real>& k) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
@@ -196,7 +204,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   const Real t3 = t2 * t1;
   const Real t5 = t3 * t2;
   if constexpr (S::vel || S::str) {
-    Real g = M::fma(Real(1.5), t5, t3);
+    Real g = M::fma(k.c15, t5, t3);
     if constexpr (!UNIF) g *= inv2 * inv;
     const Real cx = dy * s.az - dz * s.ay;
     const Real cy = dz * s.ax - dx * s.az;
@@ -212,7 +220,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
       g64[1] = fma(gd, aj64[1], g64[1]);
       g64[2] = fma(gd, aj64[2], g64[2]);
       const Real e = t.ax * cx + t.ay * cy + t.az * cz;
-      Real f2 = t5 * M::fma(Real(7.5), t2, Real(3));
+      Real f2 = t5 * M::fma(k.c75, t2, k.c3);
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
       const Real w = f2 * e;
       acc[kSd + 0] += w * dx;
@@ -243,7 +251,7 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
     acc[kW + 2] += wv * s.az;
   }
   if constexpr (S::ens) {
-    Real en = M::fma(Real(2.2), t3, t1);
+    Real en = M::fma(k.c22, t3, t1);
     if constexpr (!UNIF) en *= inv;
     acc[kE + 0] += en * s.ax;
     acc[kE + 1] += en * s.ay;
@@ -348,6 +356,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 #pragma unroll
   for (int r = 0; r < R; ++r) gsum[r][0] = gsum[r][1] = gsum[r][2] = 0.0;
 
+  PairConst<Real> kc{Real(1.5), Real(7.5), Real(2.2), Real(3)};
+  if constexpr (std::is_same<Real, float>::value)  // keep them in VGPRs
+    asm volatile("" : "+v"(kc.c15), "+v"(kc.c75), "+v"(kc.c22), "+v"(kc.c3));
+
   // ---- sweep the source tiles ------------------------------------------
   for (int t0 = lo; t0 < hi; t0 += TILE) {
     __syncthreads();
@@ -403,7 +415,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          pair_update<Real, OUT, UNIF>(part[r], gsum[r], t[r], s, aj64, eps_r2);
+          pair_update<Real, OUT, UNIF>(part[r], gsum[r], t[r], s, aj64, eps_r2, kc);
       }
       if constexpr (kFold32) {
 #pragma unroll
