@@ -28,15 +28,18 @@ from . import physics as P
 _ALL = ("vel", "stretch", "pse")
 
 
-def _opts(precision, sigma_mode, presorted=False):
+def _opts(precision, sigma_mode, presorted=False, max_splits=0):
     return L.ForwardOpts(0, P.PRECISIONS[precision],
                          L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL,
-                         1 if presorted else 0, -1)
+                         1 if presorted else 0, -1, int(max_splits))
 
 
 def adjoint(pos, alpha, sigma, nu, lam=None, mu=None, xi=None, precision="f32c",
-            sigma_mode="uniform"):
-    """dL/dx, dL/dalpha for L = <lam, u> + <mu, S> + <xi, D> (each (3,N) or None)."""
+            sigma_mode="uniform", max_splits=0):
+    """dL/dx, dL/dalpha for L = <lam, u> + <mu, S> + <xi, D> (each (3,N) or None).
+
+    ``max_splits`` caps the source splits used when the targets alone cannot
+    fill the device (0: automatic, 1: never split)."""
     p = P._particles(pos, alpha, sigma, nu if xi is not None else None)
     n = p.n
     cot = [t.contiguous() if t is not None else None for t in (lam, mu, xi)]
@@ -48,7 +51,7 @@ def adjoint(pos, alpha, sigma, nu, lam=None, mu=None, xi=None, precision="f32c",
     dpos = torch.empty((3, n), dtype=torch.float64, device=pos.device)
     dstr = torch.empty((3, n), dtype=torch.float64, device=pos.device)
     aout = L.AdjointOut(ld, dpos.data_ptr(), dstr.data_ptr())
-    opts = _opts(precision, sigma_mode)
+    opts = _opts(precision, sigma_mode, max_splits=max_splits)
     lib = L.load()
     nb = lib.vpm_adjoint_workspace_bytes(n, C.byref(opts))
     if nb == 0:

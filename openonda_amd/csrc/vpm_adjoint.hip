@@ -116,11 +116,18 @@ template <unsigned OUT>
 struct AdjSlots {
   static constexpr bool vel = OUT & VPM_OUT_VEL, str = OUT & VPM_OUT_STRETCH,
                         pse = OUT & VPM_OUT_PSE;
-  static __device__ constexpr bool used(int k) {
+  static __host__ __device__ constexpr bool used(int k) {
     return (k < 3 && (vel || str)) || (k >= 3 && k < 6 && vel) ||
            (k >= 6 && k < 9 && (vel || pse)) || (k >= 9 && k < 12 && str) ||
            (k >= 12 && k < 15) || (k >= 15 && str);
   }
+  // position of slot k among the used ones (the split partial layout)
+  static __host__ __device__ constexpr int packed(int k) {
+    int c = 0;
+    for (int q = 0; q < k; ++q) c += used(q) ? 1 : 0;
+    return c;
+  }
+  static constexpr int nslot = packed(aN);
 };
 
 template <typename Real, unsigned OUT, bool UNIF>
@@ -218,11 +225,71 @@ struct AdjArgs {
   double* dpos;
   double* dstr;
   int64_t ld;
+  // source split (see vpm_common.h): workgroup (b, k) sums sources
+  // [k*chunk, (k+1)*chunk) into part + k*nslot*pld; part == null: one split
+  double* part;
+  size_t part_cap;
+  int64_t pld;
+  int chunk;
+  int max_splits;  // 0: automatic
 };
 
+// FP64 epilogue of one sorted target k from its pair sums T (AdjSlots layout).
+template <unsigned OUT, bool UNIF>
+__device__ __forceinline__ void adj_store(const AdjArgs& a, int k, const double* T,
+                                          double scale) {
+  using S = AdjSlots<OUT>;
+  const int64_t N = a.n;
+  const AdjSorted& A = a.s;
+  const double ax = A.str[k], ay = A.str[N + k], az = A.str[2 * N + k];
+  double lx = 0, ly = 0, lz = 0, mx = 0, my = 0, mz = 0;
+  if constexpr (S::vel) {
+    lx = A.lam[k]; ly = A.lam[N + k]; lz = A.lam[2 * N + k];
+  }
+  if constexpr (S::str) {
+    mx = A.mu[k]; my = A.mu[N + k]; mz = A.mu[2 * N + k];
+  }
+  const double C = kOneOverFourPi;
+  double s2 = 1.0, s3 = 1.0, s4 = 1.0;
+  if constexpr (UNIF) {
+    s2 = scale * scale;
+    s3 = s2 * scale;
+    s4 = s2 * s2;
+  }
+  double dA[3], dX[3];
+  for (int c = 0; c < 3; ++c) {
+    dA[c] = C * s2 * T[aA + c];
+    dX[c] = C * s3 * T[aX + c];
+  }
+  const double G[3] = {T[aG], T[aG + 1], T[aG + 2]};
+  if constexpr (S::vel) {  // - C/s^3 (G x lam_m - a_m x Lambda)
+    const double Lm[3] = {T[aL], T[aL + 1], T[aL + 2]};
+    dX[0] -= C * s3 * ((G[1] * lz - G[2] * ly) - (ay * Lm[2] - az * Lm[1]));
+    dX[1] -= C * s3 * ((G[2] * lx - G[0] * lz) - (az * Lm[0] - ax * Lm[2]));
+    dX[2] -= C * s3 * ((G[0] * ly - G[1] * lx) - (ax * Lm[1] - ay * Lm[0]));
+  }
+  if constexpr (S::str) {  // dA += C/s^3 (G x mu_m + B);  dX += C/s^4 (Y x a_m + D)
+    dA[0] += C * s3 * ((G[1] * mz - G[2] * my) + T[aB]);
+    dA[1] += C * s3 * ((G[2] * mx - G[0] * mz) + T[aB + 1]);
+    dA[2] += C * s3 * ((G[0] * my - G[1] * mx) + T[aB + 2]);
+    const double Y[3] = {T[aY], T[aY + 1], T[aY + 2]};
+    dX[0] += C * s4 * ((Y[1] * az - Y[2] * ay) + T[aD]);
+    dX[1] += C * s4 * ((Y[2] * ax - Y[0] * az) + T[aD + 1]);
+    dX[2] += C * s4 * ((Y[0] * ay - Y[1] * ax) + T[aD + 2]);
+  }
+  const int64_t o = a.perm ? a.perm[k] : k;
+  for (int c = 0; c < 3; ++c) {
+    a.dpos[c * a.ld + o] = dX[c];
+    a.dstr[c * a.ld + o] = dA[c];
+  }
+}
+
+// grid (target blocks, source splits)
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int TILE, int SUB>
 __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
   using S = AdjSlots<OUT>;
+  const int lo = static_cast<int>(blockIdx.y) * a.chunk;
+  const int hi = a.chunk < a.n - lo ? lo + a.chunk : a.n;
   constexpr bool kF32 = std::is_same<Real, float>::value;
   __shared__ AdjSrc<Real> lds[TILE];
   __shared__ double red[6][BLOCK / 64];
@@ -297,13 +364,13 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     tot[q] = 0.0;
   }
 
-  for (int t0 = 0; t0 < a.n; t0 += TILE) {
+  for (int t0 = lo; t0 < hi; t0 += TILE) {
     __syncthreads();
 #pragma unroll
     for (int kk = tid; kk < TILE; kk += BLOCK) {
       const int j = t0 + kk;
       AdjSrc<Real> s;
-      if (j < a.n) {
+      if (j < hi) {
         const double sax = A.str[j], say = A.str[N + j], saz = A.str[2 * N + j];
         const double smx = A.mu[j], smy = A.mu[N + j], smz = A.mu[2 * N + j];
         s.x = static_cast<Real>((A.pos[j] - cx) * scale);
@@ -351,51 +418,67 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     for (int q = 0; q < aN; ++q) tot[q] = static_cast<double>(part[q]);
   }
 
-  // ---- epilogue (FP64) -----------------------------------------------------
-  const double C = kOneOverFourPi;
-  double s2 = 1.0, s3 = 1.0, s4 = 1.0;
-  if constexpr (UNIF) {
-    s2 = scale * scale;
-    s3 = s2 * scale;
-    s4 = s2 * s2;
+  if (a.part) {  // one split of several: raw sums, coalesced by sorted index
+    double* P = a.part + static_cast<int64_t>(blockIdx.y) * S::nslot * a.pld;
+#pragma unroll
+    for (int q = 0; q < aN; ++q)
+      if (S::used(q)) P[S::packed(q) * a.pld + k] = tot[q];
+    return;
   }
-  const double* T = tot;
-  double dA[3], dX[3];
-  for (int c = 0; c < 3; ++c) {
-    dA[c] = C * s2 * T[aA + c];
-    dX[c] = C * s3 * T[aX + c];
+  adj_store<OUT, UNIF>(a, k, tot, scale);
+}
+
+// Adds the splits' sums in split order and runs the epilogue.
+template <unsigned OUT, bool UNIF>
+__global__ __launch_bounds__(256) void adj_reduce_kernel(AdjArgs a, int nsplit) {
+  using S = AdjSlots<OUT>;
+  const int k = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  if (k >= a.n) return;
+  double T[aN];
+#pragma unroll
+  for (int q = 0; q < aN; ++q) T[q] = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) {
+    const double* P = a.part + static_cast<int64_t>(sp) * S::nslot * a.pld;
+#pragma unroll
+    for (int q = 0; q < aN; ++q)
+      if (S::used(q)) T[q] += P[S::packed(q) * a.pld + k];
   }
-  const double G[3] = {T[aG], T[aG + 1], T[aG + 2]};
-  if constexpr (S::vel) {  // - C/s^3 (G x lam_m - a_m x Lambda)
-    const double Lm[3] = {T[aL], T[aL + 1], T[aL + 2]};
-    dX[0] -= C * s3 * ((G[1] * lz - G[2] * ly) - (ay * Lm[2] - az * Lm[1]));
-    dX[1] -= C * s3 * ((G[2] * lx - G[0] * lz) - (az * Lm[0] - ax * Lm[2]));
-    dX[2] -= C * s3 * ((G[0] * ly - G[1] * lx) - (ax * Lm[1] - ay * Lm[0]));
-  }
-  if constexpr (S::str) {  // dA += C/s^3 (G x mu_m + B);  dX += C/s^4 (Y x a_m + D)
-    dA[0] += C * s3 * ((G[1] * mz - G[2] * my) + T[aB]);
-    dA[1] += C * s3 * ((G[2] * mx - G[0] * mz) + T[aB + 1]);
-    dA[2] += C * s3 * ((G[0] * my - G[1] * mx) + T[aB + 2]);
-    const double Y[3] = {T[aY], T[aY + 1], T[aY + 2]};
-    dX[0] += C * s4 * ((Y[1] * az - Y[2] * ay) + T[aD]);
-    dX[1] += C * s4 * ((Y[2] * ax - Y[0] * az) + T[aD + 1]);
-    dX[2] += C * s4 * ((Y[0] * ay - Y[1] * ax) + T[aD + 2]);
-  }
-  const int64_t o = a.perm ? a.perm[k] : k;
-  for (int c = 0; c < 3; ++c) {
-    a.dpos[c * a.ld + o] = dX[c];
-    a.dstr[c * a.ld + o] = dA[c];
-  }
+  adj_store<OUT, UNIF>(a, k, T, UNIF ? 1.0 / a.s.sig[0] : 1.0);
 }
 
 constexpr int kAdjBlock = 256, kAdjTile = 256, kAdjSub = 32;
 
 template <typename Real, unsigned OUT, bool UNIF>
-int launch_adj(const AdjArgs& a, hipStream_t st) {
-  const int blocks = static_cast<int>(ceil_div(a.n, kAdjBlock));
-  hipLaunchKernelGGL((adjoint_kernel<Real, OUT, UNIF, kAdjBlock, kAdjTile, kAdjSub>), dim3(blocks),
-                     dim3(kAdjBlock), 0, st, a);
+int launch_adj(const AdjArgs& a0, hipStream_t st) {
+  const int blocks = static_cast<int>(ceil_div(a0.n, kAdjBlock));
+  auto kern = adjoint_kernel<Real, OUT, UNIF, kAdjBlock, kAdjTile, kAdjSub>;
+  AdjArgs a = a0;
+  const int ntiles = static_cast<int>(ceil_div(a.n, kAdjTile));
+  const int64_t pld = static_cast<int64_t>(blocks) * kAdjBlock;
+  const size_t per_split = static_cast<size_t>(AdjSlots<OUT>::nslot) * pld * sizeof(double);
+  int ks = 1;
+  if (a.part) {
+    int kmax = max_splits_for(blocks, ntiles);
+    const size_t by_cap = a.part_cap / per_split;
+    if (by_cap < static_cast<size_t>(kmax)) kmax = static_cast<int>(by_cap);
+    if (a.max_splits > 0 && a.max_splits < kmax) kmax = a.max_splits;
+    if (kmax > 1) ks = choose_splits(blocks, ntiles, kmax, resident_blocks(kern, kAdjBlock));
+  }
+  if (ks > 1) {
+    a.chunk = static_cast<int>(ceil_div(ntiles, ks)) * kAdjTile;  // ks is effective
+    a.pld = pld;
+  } else {
+    a.part = nullptr;
+    a.chunk = a.n;
+    a.pld = 0;
+  }
+  hipLaunchKernelGGL(kern, dim3(blocks, ks), dim3(kAdjBlock), 0, st, a);
   VPM_LAUNCH_CHECK("adjoint_kernel");
+  if (ks > 1) {
+    hipLaunchKernelGGL((adj_reduce_kernel<OUT, UNIF>), dim3(static_cast<unsigned>(ceil_div(a.n, 256))),
+                       dim3(256), 0, st, a, ks);
+    VPM_LAUNCH_CHECK("adj_reduce_kernel");
+  }
   return VPM_OK;
 }
 
@@ -413,7 +496,8 @@ int adj_mask(const AdjArgs& a, unsigned mask, hipStream_t st) {
 }
 
 struct AdjLayout {
-  size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, sorted, cub, total, cub_bytes;
+  size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, sorted, split, cub, total, cub_bytes,
+      split_bytes;
 };
 
 int adj_layout(int64_t n, int presorted, AdjLayout* L) {
@@ -431,6 +515,13 @@ int adj_layout(int64_t n, int presorted, AdjLayout* L) {
   L->idx_in = take(N * 4);
   L->perm = take(N * 4);
   L->sorted = take(17 * N * sizeof(double));
+  // room for the largest source split, sized for all cotangents present
+  const int64_t blocks = ceil_div(n, kAdjBlock);
+  const int kmax = max_splits_for(blocks, ceil_div(n, kAdjTile));
+  L->split_bytes = kmax > 1 ? static_cast<size_t>(kmax) * AdjSlots<7u>::nslot *
+                                  static_cast<size_t>(blocks * kAdjBlock) * sizeof(double)
+                            : 0;
+  L->split = take(L->split_bytes);
   size_t cb = 0;
   if (!presorted) {
     const int rc = sort_temp_bytes(n, &cb);
@@ -572,6 +663,10 @@ int vpm_adjoint(const vpm_particles_t* p, const vpm_adjoint_in_t* in, const vpm_
   a.dpos = out->dpos;
   a.dstr = out->dstr;
   a.ld = out->ld;
+  a.part = L.split_bytes ? reinterpret_cast<double*>(ws + L.split) : nullptr;
+  a.part_cap = L.split_bytes;
+  if (o->max_splits < 0) return fail(VPM_ERR_ARG, "vpm_adjoint: max_splits < 0");
+  a.max_splits = o->max_splits;
   if (mask == 0) mask = VPM_OUT_VEL;  // all cotangents null: zero gradients from the vel pass
   const bool unif = o->sigma_mode == VPM_SIGMA_UNIFORM;
   if (o->precision == VPM_PREC_F32C)

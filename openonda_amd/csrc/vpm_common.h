@@ -54,6 +54,64 @@ int check_out(const double* out, int64_t ld, int64_t n, const char* who);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---- source splits -----------------------------------------------------
+// When the target blocks of an all-pairs pass cannot fill the device, the
+// sources are split among workgroups (grid.y) and FP64 partial sums per
+// target are added in split order by a reduce kernel (deterministic).
+constexpr int kMaxSplitBlocks = 4096;  // target blocks x splits
+
+// Most workgroups in flight at once on the current device for `kern` (queried
+// per call: kernels of one signature differ in register use).
+template <typename Kern>
+int resident_blocks(Kern kern, int block) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 1;
+  return per_cu * cus;
+}
+
+// Splits actually used when `k` are asked for: whole tiles per split, none empty.
+inline int effective_splits(int ntiles, int k) {
+  const int per = (ntiles + k - 1) / k;
+  return (ntiles + per - 1) / per;
+}
+
+// Number of source splits for `blocks` target blocks: equal-work workgroups
+// run in rounds of `resident`, so take the smallest split count (<= kmax)
+// whose launch fills its rounds best -- if that beats no split by 5%.
+inline int choose_splits(int blocks, int ntiles, int kmax, int resident) {
+  auto fill = [&](int k) {
+    const int64_t b = static_cast<int64_t>(blocks) * effective_splits(ntiles, k);
+    const int64_t rounds = (b + resident - 1) / resident;
+    return static_cast<double>(b) / static_cast<double>(rounds * resident);
+  };
+  int best = 1;
+  double best_fill = fill(1);
+  for (int k = 2; k <= kmax; ++k) {
+    const double f = fill(k);
+    if (f > best_fill) {
+      best = k;
+      best_fill = f;
+    }
+  }
+  return best_fill >= fill(1) + 0.05 ? effective_splits(ntiles, best) : 1;
+}
+
+
+// Largest split count launch code may pick for `blocks` target blocks over
+// `ntiles` source tiles (1: never split).
+inline int max_splits_for(int64_t blocks, int64_t ntiles) {
+  if (blocks <= 0 || blocks >= kMaxSplitBlocks / 2) return 1;
+  const int64_t k = ntiles < kMaxSplitBlocks / blocks ? ntiles : kMaxSplitBlocks / blocks;
+  return k > 1 ? static_cast<int>(k) : 1;
+}
+
 // Wave64 butterfly reductions.
 __device__ inline double wave_sum(double v) {
 #pragma unroll

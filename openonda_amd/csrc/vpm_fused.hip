@@ -541,50 +541,6 @@ constexpr TileCfg kCfgs[] = {
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
 
-// Most workgroups in flight at once on the current device for `kern`.
-template <typename Kern>
-int resident_blocks(Kern kern, int block) {
-  static const int per_cu = [&] {
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0) != hipSuccess) occ = 1;
-    return occ > 0 ? occ : 1;
-  }();
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus <= 0)
-    cus = 1;
-  return per_cu * cus;
-}
-
-// Splits actually used when `k` are asked for: whole tiles per split, none empty.
-int effective_splits(int ntiles, int k) {
-  const int per = (ntiles + k - 1) / k;
-  return (ntiles + per - 1) / per;
-}
-
-// Number of source splits for `blocks` target blocks: equal-work workgroups
-// run in rounds of `resident`, so take the smallest split count (<= kmax)
-// whose launch fills its rounds best -- if that beats no split by 5%.
-int choose_splits(int blocks, int ntiles, int kmax, int resident) {
-  auto fill = [&](int k) {
-    const int64_t b = static_cast<int64_t>(blocks) * effective_splits(ntiles, k);
-    const int64_t rounds = (b + resident - 1) / resident;
-    return static_cast<double>(b) / static_cast<double>(rounds * resident);
-  };
-  int best = 1;
-  double best_fill = fill(1);
-  for (int k = 2; k <= kmax; ++k) {
-    const double f = fill(k);
-    if (f > best_fill) {
-      best = k;
-      best_fill = f;
-    }
-  }
-  return best_fill >= fill(1) + 0.05 ? effective_splits(ntiles, best) : 1;
-}
-
-constexpr int kMaxSplitBlocks = 4096;
 
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
 int launch_cfg(const FusedArgs& a0, hipStream_t st, int* nblocks) {
@@ -597,9 +553,9 @@ int launch_cfg(const FusedArgs& a0, hipStream_t st, int* nblocks) {
   const int64_t pld = static_cast<int64_t>(blocks) * BLOCK * R;
   const size_t per_split = static_cast<size_t>(Slots<OUT>::nslot) * pld * sizeof(double);
   int ks = 1;
-  if (a.part && blocks < kMaxSplitBlocks / 2) {
+  if (a.part) {
     const size_t by_cap = a.part_cap / per_split;
-    int kmax = std::min(ntiles, kMaxSplitBlocks / blocks);
+    int kmax = max_splits_for(blocks, ntiles);
     if (by_cap < static_cast<size_t>(kmax)) kmax = static_cast<int>(by_cap);
     if (a.max_splits > 0 && a.max_splits < kmax) kmax = a.max_splits;
     if (kmax > 1) ks = choose_splits(blocks, ntiles, kmax, resident_blocks(kern, BLOCK));
@@ -708,8 +664,7 @@ size_t split_capacity(int64_t n, const vpm_forward_opts_t* o) {
     tile = kCfgs[c].tile;
   }
   const int64_t blocks = ceil_div(n, br);
-  if (blocks >= kMaxSplitBlocks / 2) return 0;
-  const int64_t kmax = std::min<int64_t>(ceil_div(n, tile), kMaxSplitBlocks / blocks);
+  const int kmax = max_splits_for(blocks, ceil_div(n, tile));
   if (kmax < 2) return 0;
   return static_cast<size_t>(kmax) * split_slots(kernel_mask(o->outputs)) *
          static_cast<size_t>(blocks * br) * sizeof(double);

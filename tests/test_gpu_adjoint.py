@@ -163,3 +163,20 @@ def test_rollout_gradient_vs_torch_autograd(gpu):
     uu = AG.probe_velocity(P.to_soa(y, dev), xs, as_, sig)
     (lam.t() * uu).sum().backward()
     assert rel(px.grad.t(), rx) <= 1e-10 and rel(pa.grad.t(), ra) <= 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-13), ("f32c", 2e-7)])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_adjoint_source_split_matches_single_pass(gpu, precision, tol, uniform):
+    """Sources split among workgroups (small N) give the unsplit gradient."""
+    x, a, s, nu = make(3000, 21, uniform)
+    print(f"[adjoint-split] {precision} uniform={uniform}", flush=True)
+    d = [P.to_soa(x), P.to_soa(a), torch.from_numpy(s).cuda(), torch.from_numpy(nu).cuda()]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    cots = [torch.randn(3, len(x), dtype=torch.float64, device="cuda", generator=g) for _ in range(3)]
+    mode = "uniform" if uniform else "general"
+    one = AG.adjoint(*d, *cots, precision=precision, sigma_mode=mode, max_splits=1)
+    auto = AG.adjoint(*d, *cots, precision=precision, sigma_mode=mode)
+    two = AG.adjoint(*d, *cots, precision=precision, sigma_mode=mode, max_splits=2)
+    for ref, got_a, got_2 in zip(one, auto, two):
+        assert rel(got_a, ref) <= tol and rel(got_2, ref) <= tol
