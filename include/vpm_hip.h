@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VPM_ABI_VERSION 2
+#define VPM_ABI_VERSION 3
 
 #define VPM_OK 0
 #define VPM_ERR_ARG (-1)        /* bad shape, stride, option or null pointer */
@@ -153,7 +153,19 @@ typedef struct vpm_forward_opts {
                          * target blocks alone cannot fill the device, the sources
                          * are split among workgroups and the FP64 partial sums
                          * are added in split order (deterministic). */
+  int32_t g64_mode;     /* VPM_G64_*: how the FP32 pass accumulates the stretching's
+                         * sum_j g alpha_j for the output sets VEL|STRETCH and
+                         * STRETCH (with PSE, VORT or ENSTROPHY it is always FP64;
+                         * ignored by the FP64 pass) */
 } vpm_forward_opts_t;
+
+/* VPM_G64_AUTO: FP64 per pair in waves with a row of 64 consecutive sorted
+ * targets whose strengths are coherent (|sum a_i| > 0.45 sum |a_i|: rings,
+ * tubes, sheets, where a_i x G nearly cancels), FP32 partials with FP64
+ * folds elsewhere. */
+#define VPM_G64_AUTO 0
+#define VPM_G64_ALWAYS 1
+#define VPM_G64_NEVER 2
 
 typedef struct vpm_forward_out {
   int64_t ld;           /* component stride of the vector outputs, >= n */

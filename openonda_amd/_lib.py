@@ -34,6 +34,10 @@ PREC_F32C = 1
 SIGMA_GENERAL = 0
 SIGMA_UNIFORM = 1
 
+G64_AUTO = 0
+G64_ALWAYS = 1
+G64_NEVER = 2
+
 
 class VpmError(RuntimeError):
     """A libvpm_hip call returned a non-zero status."""
@@ -54,7 +58,8 @@ class Points(C.Structure):
 
 class ForwardOpts(C.Structure):
     _fields_ = [("outputs", C.c_uint32), ("precision", C.c_int32), ("sigma_mode", C.c_int32),
-                ("presorted", C.c_int32), ("tile_config", C.c_int32), ("max_splits", C.c_int32)]
+                ("presorted", C.c_int32), ("tile_config", C.c_int32), ("max_splits", C.c_int32),
+                ("g64_mode", C.c_int32)]
 
 
 class ForwardOut(C.Structure):
@@ -147,7 +152,7 @@ def load() -> C.CDLL:
             fn = getattr(lib, name)  # AttributeError = stale library
             fn.restype = res
             fn.argtypes = args
-        if lib.vpm_abi_version() != 2:
+        if lib.vpm_abi_version() != 3:
             raise RuntimeError("libvpm_hip.so ABI version mismatch")
         _lib = lib
         return lib

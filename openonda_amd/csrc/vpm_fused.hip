@@ -55,6 +55,7 @@ struct FusedArgs {
   double* part;
   size_t part_cap;      // bytes available at part
   int max_splits;       // 0: automatic
+  int g64_mode;         // VPM_G64_*: per-pair FP64 alpha_j sum of the stretching
   int64_t pld;
   int chunk;
 };
@@ -100,10 +101,12 @@ struct Slots {
   static constexpr bool vel = OUT & VPM_OUT_VEL, str = OUT & VPM_OUT_STRETCH,
                         pse = OUT & VPM_OUT_PSE, vort = OUT & VPM_OUT_VORT,
                         ens = OUT & VPM_OUT_ENSTROPHY;
-  // FP32-accumulated slots.  kG (the alpha_j sum of the stretching) is
-  // accumulated in FP64 per pair instead: on symmetric vortex structures the
-  // term alpha_i x G is a near-exact cancellation and FP32 partials of G
-  // cost an order of magnitude in the stretching's relative error.
+  // Slots with FP64 totals.  The stretching's alpha_j sum G has FP64
+  // totals of its own (gsum): waves whose targets' strengths are coherent
+  // (vortex rings, tubes, sheets) add to it in FP64 per pair, because there
+  // alpha_i x G is a near-exact cancellation and FP32 partials of G cost an
+  // order of magnitude in the stretching's relative error (coherent_wave
+  // below); other waves fold FP32 partials kG..kG+2 into it per sub-tile.
   static __host__ __device__ constexpr bool used(int k) {
     return (k < 3 && vel) || (k >= 6 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
            (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
@@ -168,7 +171,7 @@ struct PairConst {
   Real c15, c75, c22, c3;
 };
 
-template <typename Real, unsigned OUT, bool UNIF>
+template <typename Real, unsigned OUT, bool UNIF, bool G64>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
                                             const SrcR<Real, UNIF>& s, const double* aj64,
                                             Real eps_r2, const PairConst<Real>& k) {
@@ -215,10 +218,16 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
       acc[kU + 2] += g * cz;
     }
     if constexpr (S::str) {
-      const double gd = static_cast<double>(g);
-      g64[0] = fma(gd, aj64[0], g64[0]);
-      g64[1] = fma(gd, aj64[1], g64[1]);
-      g64[2] = fma(gd, aj64[2], g64[2]);
+      if constexpr (G64) {
+        const double gd = static_cast<double>(g);
+        g64[0] = fma(gd, aj64[0], g64[0]);
+        g64[1] = fma(gd, aj64[1], g64[1]);
+        g64[2] = fma(gd, aj64[2], g64[2]);
+      } else {
+        acc[kG + 0] += g * s.ax;
+        acc[kG + 1] += g * s.ay;
+        acc[kG + 2] += g * s.az;
+      }
       const Real e = t.ax * cx + t.ay * cy + t.az * cz;
       Real f2 = t5 * M::fma(k.c75, t2, k.c3);
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
@@ -257,6 +266,35 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
     acc[kE + 1] += en * s.ay;
     acc[kE + 2] += en * s.az;
   }
+}
+
+// Coherence of 64 consecutive sorted targets (one lane row of a wave),
+// |sum alpha_i| / sum |alpha_i|: for unrelated strengths it is ~0.14 and
+// stayed below 0.35 over the 16384 rows of a random cloud of 2^20 particles;
+// along vortex rings, tubes and sheets (the example cases, leapfrogging
+// rings) it stayed above 0.59.  A wave with a row above kCoherent sums G in
+// FP64 per pair.
+constexpr double kCoherent = 0.45;
+
+template <int R>
+__device__ __forceinline__ bool coherent_wave(const Src8* tg, const bool* valid) {
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double sx = 0.0, sy = 0.0, sz = 0.0, m = 0.0;
+    if (valid[r]) {
+      sx = tg[r].ax;
+      sy = tg[r].ay;
+      sz = tg[r].az;
+      m = sqrt(sx * sx + sy * sy + sz * sz);
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    sz = wave_sum(sz);
+    m = wave_sum(m);
+    any = any || sqrt(sx * sx + sy * sy + sz * sz) > kCoherent * m;
+  }
+  return any;
 }
 
 // One workgroup's work: BLOCK*R consecutive sorted targets of a particle set
@@ -338,6 +376,16 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     t[r].hs = static_cast<Real>(0.5 * tg[r].s);
   }
   const Real eps_r2 = static_cast<Real>(eps_r2_d);
+  // Wave-uniform choice of the G accumulation (FP32 passes).  Only the
+  // velocity/stretching output sets carry both loops: with PSE, vorticity or
+  // enstrophy the second loop costs occupancy, and G is summed in FP64.
+  constexpr bool kBothG = S::str && !S::pse && !S::vort && !S::ens;
+  bool g64 = S::str && std::is_same<Real, float>::value;
+  if constexpr (kBothG && std::is_same<Real, float>::value) {
+    g64 = a.g64_mode == VPM_G64_ALWAYS ||
+          (a.g64_mode == VPM_G64_AUTO && coherent_wave<R>(tg, valid));
+    g64 = __builtin_amdgcn_readfirstlane(static_cast<int>(g64)) != 0;
+  }
 
   Real part[R][kNAcc];
   Real ksum[R][kNAcc];
@@ -404,22 +452,34 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     __syncthreads();
 
     for (int sb = 0; sb < TILE; sb += SUB) {
+      if (g64) {  // constant unless kBothG
 #pragma unroll 8
-      for (int q = 0; q < SUB; ++q) {
-        const SrcR<Real, UNIF> s = lds[sb + q];
-        double aj64[3] = {0.0, 0.0, 0.0};
-        if constexpr (S::str) {
-          aj64[0] = lds_a64[sb + q][0];
-          aj64[1] = lds_a64[sb + q][1];
-          aj64[2] = lds_a64[sb + q][2];
-        }
+        for (int q = 0; q < SUB; ++q) {
+          const SrcR<Real, UNIF> s = lds[sb + q];
+          const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1], lds_a64[sb + q][2]};
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          pair_update<Real, OUT, UNIF>(part[r], gsum[r], t[r], s, aj64, eps_r2, kc);
+          for (int r = 0; r < R; ++r)
+            pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, eps_r2, kc);
+        }
+      } else {
+#pragma unroll 8
+        for (int q = 0; q < SUB; ++q) {
+          const SrcR<Real, UNIF> s = lds[sb + q];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, eps_r2, kc);
+        }
       }
       if constexpr (kFold32) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int r = 0; r < R; ++r) {
+          if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              gsum[r][c] += static_cast<double>(part[r][kG + c]);
+              part[r][kG + c] = Real(0);
+            }
+          }
 #pragma unroll
           for (int q = 0; q < kNAcc; ++q) {
             if (!S::used(q)) continue;
@@ -433,6 +493,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
             }
             part[r][q] = Real(0);
           }
+        }
       }
     }
     if constexpr (kComp) {
@@ -449,9 +510,14 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   }
   if constexpr (!kFold32) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int q = 0; q < kNAcc; ++q) tot[r][q] = static_cast<double>(part[r][q]);
+      if constexpr (S::str) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gsum[r][c] += static_cast<double>(part[r][kG + c]);
+      }
+    }
   }
 
   // ---- epilogue ----------------------------------------------------------
@@ -1039,6 +1105,10 @@ int check_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, const c
     set_error("%s: max_splits < 0", who);
     return VPM_ERR_ARG;
   }
+  if (o->g64_mode < VPM_G64_AUTO || o->g64_mode > VPM_G64_NEVER) {
+    set_error("%s: unknown g64_mode", who);
+    return VPM_ERR_ARG;
+  }
   return check_particles(p, (want & VPM_OUT_PSE) != 0, who);
 }
 
@@ -1133,6 +1203,7 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   a.part = L.split_bytes ? reinterpret_cast<double*>(ws + L.split) : nullptr;
   a.part_cap = L.split_bytes;
   a.max_splits = o->max_splits;
+  a.g64_mode = o->g64_mode;
   int nblocks = 0;
   const int cfg = o->tile_config < 0 ? 0 : o->tile_config;
   if ((rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st,
@@ -1211,6 +1282,8 @@ int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm
     return fail(VPM_ERR_ARG, "vpm_batch_run: unknown precision");
   if (o->tile_config < -1 || o->tile_config >= kNumCfgs)
     return fail(VPM_ERR_ARG, "vpm_batch_run: tile_config out of range");
+  if (o->g64_mode < VPM_G64_AUTO || o->g64_mode > VPM_G64_NEVER)
+    return fail(VPM_ERR_ARG, "vpm_batch_run: unknown g64_mode");
   if ((want & VPM_OUT_PSE) && p->n && !p->nu) return fail(VPM_ERR_ARG, "vpm_batch_run: PSE needs nu");
   const int64_t n = p->n;
   if (out->ld < n) return fail(VPM_ERR_ARG, "vpm_batch_run: out->ld < n");
@@ -1240,6 +1313,7 @@ int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm
   route(VPM_OUT_PSE, out->pse, &b.f.pse, &b.f.ld_pse);
   route(VPM_OUT_VORT, out->vort, &b.f.vort, &b.f.ld_vort);
   b.f.ens_part = reinterpret_cast<double*>(ws + L.ens_part);
+  b.f.g64_mode = o->g64_mode;
   b.src = reinterpret_cast<const Src8*>(ws + L.src8);
   b.perm = reinterpret_cast<const int*>(ws + L.perm);
   b.inst_off = d->set_off;

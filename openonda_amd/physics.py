@@ -20,6 +20,7 @@ from . import _lib as L
 
 VARIANTS = {"cuda": L.VARIANT_CUDA, "cpu": L.VARIANT_CPU}
 PRECISIONS = {"f64": L.PREC_F64, "f32c": L.PREC_F32C}
+G64_MODES = {"auto": L.G64_AUTO, "always": L.G64_ALWAYS, "never": L.G64_NEVER}
 OUTPUTS = {"vel": L.OUT_VEL, "stretch": L.OUT_STRETCH, "pse": L.OUT_PSE, "vort": L.OUT_VORT,
            "enstrophy": L.OUT_ENSTROPHY}
 
@@ -196,7 +197,8 @@ def sigma_is_uniform(sigma: torch.Tensor) -> bool:
 
 
 def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f32c",
-            sigma_mode="auto", presorted=False, tile_config=-1, max_splits=0, out=None):
+            sigma_mode="auto", presorted=False, tile_config=-1, max_splits=0, g64="auto",
+            out=None):
     """One fused all-pairs pass (GPU-plugin semantics) returning a dict.
 
     ``outputs`` is any subset of {"vel", "stretch", "pse", "vort",
@@ -209,6 +211,9 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
     ``max_splits``: 0 lets the library split the sources among workgroups
     when the targets alone cannot fill the device (small N); k >= 1 caps
     the number of splits (1: never split).
+    ``g64``: how the "f32c" pass sums the stretching's g alpha_j -- "auto"
+    (FP64 per pair in 64-lane waves of coherent target strengths, FP32
+    partials elsewhere), "always" or "never".
     ``out`` may supply preallocated tensors by name.
     """
     if isinstance(outputs, str):
@@ -237,7 +242,8 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
             if precision == "f32c" else -1
     opts = L.ForwardOpts(mask, PRECISIONS[precision],
                          L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL,
-                         1 if presorted else 0, int(tile_config), int(max_splits))
+                         1 if presorted else 0, int(tile_config), int(max_splits),
+                         G64_MODES[g64])
     out = dict(out or {})
     res = {}
     ld = None

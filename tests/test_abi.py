@@ -39,7 +39,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_load_and_version():
     lib = L.load()
-    assert lib.vpm_abi_version() == 2
+    assert lib.vpm_abi_version() == 3
     assert lib.vpm_device_count() >= 0
     assert lib.vpm_last_error() is not None
 

@@ -202,3 +202,33 @@ def test_source_split_matches_single_pass(gpu, n, precision, mode, outputs):
             assert err <= tol, (name, k, err)
     ref = oracle_all(x, a, s, nu)
     check({k: auto[k] for k in outputs}, ref, F64_TOL if precision == "f64" else F32C_TOL, "auto")
+
+
+def test_g64_modes_on_ring_and_cloud(gpu):
+    """The stretching's alpha_j sum: FP64 per pair in coherent waves (a ring),
+    FP32 partials in incoherent ones (a random cloud); both within the bar."""
+    outputs = ("vel", "stretch")
+    xr, ar, sr, nur = ring_set()
+    xc, ac, sc, nuc = FM.random_cloud(8192, seed=17)
+    ref_r = cached("ring", lambda: oracle_all(xr, ar, sr, nur))
+    ref_c = {"vel": O.selfinduced_velocity(xc, ac, sc), "stretch": O.strength_gradients(xc, ac, sc)}
+    runs = {}
+    for name, (x, a, s, nu) in (("ring", (xr, ar, sr, nur)), ("cloud", (xc, ac, sc, nuc))):
+        d = to_dev(x, a, s, nu)
+        for mode in ("auto", "always", "never"):
+            print(f"[g64] {name} {mode}", flush=True)
+            runs[name, mode] = P.forward(*d, outputs=outputs, precision="f32c",
+                                         sigma_mode="uniform", g64=mode)
+    # a ring's waves are coherent: automatic = FP64 per pair, bit for bit
+    for k in outputs:
+        assert torch.equal(runs["ring", "auto"][k], runs["ring", "always"][k])
+        assert torch.equal(runs["cloud", "auto"][k], runs["cloud", "never"][k])
+    check(runs["ring", "auto"], ref_r, F32C_TOL, "ring auto")
+    check(runs["cloud", "auto"], ref_c, F32C_TOL, "cloud auto")
+    check(runs["cloud", "always"], ref_c, F32C_TOL, "cloud always")
+    # FP32 partials on the ring lose accuracy: the reason for the FP64 path
+    err_never = rel_l2(P.from_soa(runs["ring", "never"]["stretch"]), ref_r["stretch"])
+    err_auto = rel_l2(P.from_soa(runs["ring", "auto"]["stretch"]), ref_r["stretch"])
+    assert err_never > 2 * err_auto
+    with pytest.raises(KeyError):
+        P.forward(*to_dev(xr, ar, sr, nur), outputs=outputs, g64="sometimes")
