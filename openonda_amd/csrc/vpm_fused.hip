@@ -597,12 +597,14 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(FusedArgs a, int nspl
 struct TileCfg {
   int block, r, tile, sub, comp;
 };
-// Index 0 is the FP32 default (the autotuner's winner at N = 2^20: direct
-// FP64 fold of 32-source FP32 partials).  Configs 1.. exist for the hot output sets
+// Index 0 is the FP32 default (the autotuner's winner for velocity +
+// stretching at N = 2^20: one target per lane, 64-source FP32 partials into a
+// Kahan pair per tile).  Configs 1.. exist for the hot output sets
 // (VEL|STRETCH, VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
 constexpr TileCfg kCfgs[] = {
-    {256, 2, 256, 32, 1}, {256, 4, 256, 32, 0}, {256, 2, 256, 32, 0}, {256, 4, 256, 32, 1},
-    {128, 4, 256, 32, 0}, {256, 1, 256, 64, 0}, {512, 2, 512, 32, 0}, {256, 2, 512, 64, 0},
+    {256, 1, 256, 64, 0},  {256, 2, 256, 32, 1}, {256, 2, 256, 32, 0}, {256, 1, 256, 64, 1},
+    {256, 1, 256, 128, 0}, {256, 1, 512, 64, 0}, {512, 1, 512, 64, 0}, {128, 1, 256, 64, 0},
+    {256, 2, 512, 64, 0},  {256, 4, 256, 32, 0},
 };
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
@@ -660,6 +662,7 @@ int launch_f32(const FusedArgs& a, int cfg, hipStream_t st, int* nb) {
   if constexpr (UNIF && (OUT == 3u || OUT == 7u || OUT == 27u)) {
     switch (cfg) {
       VPM_CFG(1) VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7)
+      VPM_CFG(8) VPM_CFG(9)
       default:
         break;
     }
@@ -968,6 +971,7 @@ int launch_batch_mask(const BatchArgs& b, int prec, int cfg, int grid, hipStream
                             kCfgs[I].sub, kCfgs[I].comp>(b, grid, st);
   switch (cfg) {
     VPM_BCFG(1) VPM_BCFG(2) VPM_BCFG(3) VPM_BCFG(4) VPM_BCFG(5) VPM_BCFG(6) VPM_BCFG(7)
+    VPM_BCFG(8) VPM_BCFG(9)
     default:
       break;
   }

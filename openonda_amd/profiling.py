@@ -40,7 +40,7 @@ def kernel_stats(db: str):
     for name, calls, total, pct in rows:
         e = agg.setdefault(_short(name), [0, 0.0, 0.0])
         e[0] += calls
-        e[1] += total / 1e3  # ns -> us
+        e[1] += total  # the view reports microseconds
         e[2] += pct
     out = [(k, c, t, t / c, p) for k, (c, t, p) in agg.items()]
     return sorted(out, key=lambda r: -r[2])

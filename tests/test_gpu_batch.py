@@ -63,7 +63,7 @@ def test_batch_forward_matches_per_set(gpu, precision, tol):
         assert float(bs.enstrophy[k]) == pytest.approx(O.total_enstrophy(x, a, sg), rel=10 * tol)
 
 
-@pytest.mark.parametrize("cfg", range(8))
+@pytest.mark.parametrize("cfg", range(10))
 def test_batch_tile_configs(gpu, cfg):
     sets = small_sets(1)
     bs = B.BatchedSystem(sets, dt=0.01, precision="f32c", tile_config=cfg)
