@@ -130,9 +130,16 @@ struct AdjSlots {
   static constexpr int nslot = packed(aN);
 };
 
+// Non-inline constants of the pair terms, kept in registers by the caller
+// (literal operands halve an instruction's issue rate on gfx950).
+template <typename Real>
+struct AdjConst {
+  Real c15, c3, c75, c2625, c105, c945;
+};
+
 template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const AdjSrc<Real>& s,
-                                         Real sc) {
+                                         Real sc, const AdjConst<Real>& k) {
   using S = AdjSlots<OUT>;
   using M = Ar<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
@@ -144,14 +151,16 @@ __device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const
     inv2 = inv * inv;
     A = (dx * dx + dy * dy + dz * dz) * inv2 + Real(1);
   }
+  // A t^2 = 1: each factor (rho^2 + c) t^k is written in powers of t alone
   const Real t1 = M::rsq(A);
   const Real t2 = t1 * t1;
-  const Real t5 = t2 * t2 * t1;
+  const Real t3 = t2 * t1;
+  const Real t5 = t3 * t2;
   const Real t7 = t5 * t2;
   Real xc = Real(0);  // radial coefficient of aX
   if constexpr (S::vel || S::str) {
-    Real g = (A + Real(1.5)) * t5;                   // (rho^2 + 2.5) t^5
-    Real f = (Real(3) * A + Real(7.5)) * t7;         // 3 (rho^2 + 3.5) t^7
+    Real g = k.c15 * t5 + t3;                        // (rho^2 + 2.5) t^5
+    Real f = t5 * (k.c75 * t2 + k.c3);               // 3 (rho^2 + 3.5) t^7
     if constexpr (!UNIF) {
       const Real inv3 = inv2 * inv;
       g *= inv3;
@@ -177,7 +186,7 @@ __device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const
       xc = f * e;
     }
     if constexpr (S::str) {
-      Real h = (Real(7.5) * A + Real(26.25)) * (t7 * t2);  // 7.5 (rho^2 + 4.5) t^9
+      Real h = t7 * (k.c2625 * t2 + k.c75);            // 7.5 (rho^2 + 4.5) t^9
       if constexpr (!UNIF) h *= inv2 * inv2 * inv2 * inv;
       const Real dmx = t.mx - s.mx, dmy = t.my - s.my, dmz = t.mz - s.mz;
       const Real phi = dmx * dx + dmy * dy + dmz * dz;
@@ -205,13 +214,13 @@ __device__ __forceinline__ void adj_pair(Real* acc, const AdjTgt<Real>& t, const
       q *= inv2 * inv2;
     }
     const Real zx = s.zx - t.zx, zy = s.zy - t.zy, zz = s.zz - t.zz;
-    const Real p105 = Real(105) * p;
+    const Real p105 = k.c105 * p;
     acc[aA + 0] += p105 * zx;
     acc[aA + 1] += p105 * zy;
     acc[aA + 2] += p105 * zz;
     // (a_j - a_m).(z_m - z_j) = -(a_j - a_m).dz
     const Real dd = (s.ax - t.ax) * zx + (s.ay - t.ay) * zy + (s.az - t.az) * zz;
-    xc += Real(945) * q * dd;
+    xc += k.c945 * q * dd;
   }
   acc[aX + 0] += xc * dx;
   acc[aX + 1] += xc * dy;
@@ -364,6 +373,11 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     tot[q] = 0.0;
   }
 
+  AdjConst<Real> kc{Real(1.5), Real(3), Real(7.5), Real(26.25), Real(105), Real(945)};
+  if constexpr (kF32)  // keep them in VGPRs
+    asm volatile("" : "+v"(kc.c15), "+v"(kc.c3), "+v"(kc.c75), "+v"(kc.c2625), "+v"(kc.c105),
+                 "+v"(kc.c945));
+
   for (int t0 = lo; t0 < hi; t0 += TILE) {
     __syncthreads();
 #pragma unroll
@@ -400,7 +414,7 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
     if (valid) {
       for (int sb = 0; sb < TILE; sb += SUB) {
 #pragma unroll 4
-        for (int q = 0; q < SUB; ++q) adj_pair<Real, OUT, UNIF>(part, t, lds[sb + q], rsc);
+        for (int q = 0; q < SUB; ++q) adj_pair<Real, OUT, UNIF>(part, t, lds[sb + q], rsc, kc);
         if constexpr (kF32) {
 #pragma unroll
           for (int q = 0; q < aN; ++q) {
