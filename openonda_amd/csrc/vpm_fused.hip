@@ -88,9 +88,10 @@ struct alignas(16) SrcR<Real, false> {
   Real x, y, z, ax, ay, az, hs, is2, is3, p0, p1, p2;
 };
 
+// (lx, ly, lz): the low parts alpha - fp32(alpha) of an FP32 pass with PSE.
 template <typename Real>
 struct Tgt {
-  Real x, y, z, ax, ay, az, hs;
+  Real x, y, z, ax, ay, az, hs, lx, ly, lz;
 };
 
 // Accumulator slots.
@@ -171,10 +172,13 @@ struct PairConst {
   Real c15, c75, c22, c3;
 };
 
+// slo: the source's low parts (lx, ly, lz) when the PSE difference is taken
+// in two parts (FP32 pass), else null.
 template <typename Real, unsigned OUT, bool UNIF, bool G64>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
                                             const SrcR<Real, UNIF>& s, const double* aj64,
-                                            Real eps_r2, const PairConst<Real>& k) {
+                                            const Real* slo, Real eps_r2,
+                                            const PairConst<Real>& k) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
@@ -240,9 +244,18 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   if constexpr (S::pse) {
     Real t9 = t5 * t2 * t2;
     if constexpr (!UNIF) t9 *= inv2;
-    acc[kP + 0] += t9 * (s.ax - t.ax);
-    acc[kP + 1] += t9 * (s.ay - t.ay);
-    acc[kP + 2] += t9 * (s.az - t.az);
+    Real dax = s.ax - t.ax, day = s.ay - t.ay, daz = s.az - t.az;
+    if constexpr (std::is_same<Real, float>::value) {
+      // alpha_j - alpha_i from high and low parts: the high difference is
+      // exact when the strengths are close (smooth fields), where a rounded
+      // FP32 strength would lose the small difference
+      dax += slo[0] - t.lx;
+      day += slo[1] - t.ly;
+      daz += slo[2] - t.lz;
+    }
+    acc[kP + 0] += t9 * dax;
+    acc[kP + 1] += t9 * day;
+    acc[kP + 2] += t9 * daz;
   }
   if constexpr (S::vort) {
     Real wv;
@@ -311,8 +324,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
   constexpr bool kFold32 = std::is_same<Real, float>::value;
+  constexpr bool kLo = S::pse && std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
+  __shared__ float4 lds_lo[kLo ? TILE : 1];
   __shared__ double red[6][BLOCK / 64];
   __shared__ double centre[3];
 
@@ -374,6 +389,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     t[r].ay = static_cast<Real>(tg[r].ay);
     t[r].az = static_cast<Real>(tg[r].az);
     t[r].hs = static_cast<Real>(0.5 * tg[r].s);
+    t[r].lx = t[r].ly = t[r].lz = Real(0);
+    if constexpr (kLo) {
+      t[r].lx = static_cast<Real>(tg[r].ax - static_cast<double>(t[r].ax));
+      t[r].ly = static_cast<Real>(tg[r].ay - static_cast<double>(t[r].ay));
+      t[r].lz = static_cast<Real>(tg[r].az - static_cast<double>(t[r].az));
+    }
   }
   const Real eps_r2 = static_cast<Real>(eps_r2_d);
   // Wave-uniform choice of the G accumulation (FP32 passes).  Only the
@@ -408,6 +429,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   if constexpr (std::is_same<Real, float>::value)  // keep them in VGPRs
     asm volatile("" : "+v"(kc.c15), "+v"(kc.c75), "+v"(kc.c22), "+v"(kc.c3));
 
+  // Sources per unrolled step: deep unrolling lets the compiler prefetch LDS
+  // records for 8 sources, which the heaviest output sets pay for in occupancy.
+  constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
+
   // ---- sweep the source tiles ------------------------------------------
   for (int t0 = lo; t0 < hi; t0 += TILE) {
     __syncthreads();
@@ -428,6 +453,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           lds_a64[k][1] = g.ay;
           lds_a64[k][2] = g.az;
         }
+        if constexpr (kLo)
+          lds_lo[k] = make_float4(static_cast<float>(g.ax - static_cast<double>(s.ax)),
+                                  static_cast<float>(g.ay - static_cast<double>(s.ay)),
+                                  static_cast<float>(g.az - static_cast<double>(s.az)), 0.0f);
         if constexpr (!UNIF) {
           s.hs = static_cast<Real>(0.5 * g.s);
           s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
@@ -441,6 +470,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           lds_a64[k][1] = 0.0;
           lds_a64[k][2] = 0.0;
         }
+        if constexpr (kLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (!UNIF) {
           s.hs = Real(0.5);
           s.is2 = Real(1);
@@ -453,21 +483,36 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 
     for (int sb = 0; sb < TILE; sb += SUB) {
       if (g64) {  // constant unless kBothG
-#pragma unroll 8
+#pragma unroll kUnroll
         for (int q = 0; q < SUB; ++q) {
           const SrcR<Real, UNIF> s = lds[sb + q];
           const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1], lds_a64[sb + q][2]};
+          Real slo[3] = {Real(0), Real(0), Real(0)};
+          if constexpr (kLo) {
+            const float4 l = lds_lo[sb + q];
+            slo[0] = l.x;
+            slo[1] = l.y;
+            slo[2] = l.z;
+          }
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, eps_r2, kc);
+            pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2, kc);
         }
       } else {
-#pragma unroll 8
+#pragma unroll kUnroll
         for (int q = 0; q < SUB; ++q) {
           const SrcR<Real, UNIF> s = lds[sb + q];
+          Real slo[3] = {Real(0), Real(0), Real(0)};
+          if constexpr (kLo) {
+            const float4 l = lds_lo[sb + q];
+            slo[0] = l.x;
+            slo[1] = l.y;
+            slo[2] = l.z;
+          }
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, eps_r2, kc);
+            pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, slo, eps_r2,
+                                                kc);
         }
       }
       if constexpr (kFold32) {

@@ -232,3 +232,15 @@ def test_g64_modes_on_ring_and_cloud(gpu):
     assert err_never > 2 * err_auto
     with pytest.raises(KeyError):
         P.forward(*to_dev(xr, ar, sr, nur), outputs=outputs, g64="sometimes")
+
+
+def test_pse_accuracy_on_coaxial_rings(gpu):
+    """PSE on a smooth field: alpha_j - alpha_i from high and low FP32 parts
+    keeps the FP32 pass within the bar (one rounded FP32 strength per
+    particle would lose the small differences)."""
+    x, a, s, nu = FM.leapfrog_rings(n_stations=64, nu=1e-3)
+    print(f"[pse-rings] n={len(x)}", flush=True)
+    ref = O.strength_gradients_pse(x, a, s, nu)
+    res = P.forward(*to_dev(x, a, s, nu), outputs=("vel", "stretch", "pse"), precision="f32c",
+                    sigma_mode="uniform")
+    assert rel_l2(P.from_soa(res["pse"]), ref) <= F32C_TOL
