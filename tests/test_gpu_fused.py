@@ -234,12 +234,16 @@ def test_g64_modes_on_ring_and_cloud(gpu):
         P.forward(*to_dev(xr, ar, sr, nur), outputs=outputs, g64="sometimes")
 
 
-def test_pse_accuracy_on_coaxial_rings(gpu):
-    """PSE on a smooth field: alpha_j - alpha_i from high and low FP32 parts
-    keeps the FP32 pass within the bar (one rounded FP32 strength per
-    particle would lose the small differences)."""
-    x, a, s, nu = FM.leapfrog_rings(n_stations=64, nu=1e-3)
-    print(f"[pse-rings] n={len(x)}", flush=True)
+def test_pse_accuracy_on_nearly_uniform_strengths(gpu):
+    """PSE sums eta (alpha_j - alpha_i); on a nearly uniform field the
+    differences are 1e-3 of the strengths.  Taking them from high and low
+    FP32 parts keeps the FP32 pass within the bar, where one rounded FP32
+    strength per particle would leave ~1e-4."""
+    x, _, s, _ = FM.random_cloud(4096, seed=31)
+    rng = np.random.default_rng(32)
+    a = np.array([1.0, -0.5, 0.25]) / 4096 * (1.0 + 1e-3 * rng.standard_normal((4096, 3)))
+    nu = np.full(4096, 1e-3)
+    print("[pse-uniform] n=4096", flush=True)
     ref = O.strength_gradients_pse(x, a, s, nu)
     res = P.forward(*to_dev(x, a, s, nu), outputs=("vel", "stretch", "pse"), precision="f32c",
                     sigma_mode="uniform")
