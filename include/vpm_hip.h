@@ -195,6 +195,17 @@ int vpm_tile_config_count(void);
 int vpm_tile_config_info(int index, int* block, int* targets_per_lane, int* tile,
                          int* subtile, int* comp);
 
+/* Autotuning (no counterpart in the reference): sorts once, then times every
+ * tile configuration of the VPM_PREC_F32C pass `reps` times on the caller's
+ * particles with HIP events, and returns the fastest index in *best and, if
+ * ms is not NULL, each configuration's mean time in ms[0..count).  The
+ * workspace must hold vpm_autotune_workspace_bytes (the largest layout).
+ * Synchronises the stream.  (openonda_amd/autotune.py adds an accuracy check
+ * against the FP64 pass and a cache keyed by N.) */
+size_t vpm_autotune_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts);
+int vpm_autotune(const vpm_particles_t* p, const vpm_forward_opts_t* opts, vpm_forward_out_t* out,
+                 void* workspace, size_t ws_bytes, void* stream, int reps, int* best, float* ms);
+
 /* ---- batched persistent launch ------------------------------------------
  * No counterpart in the reference (one particle set per call there).  Many
  * particle sets of uneven size are concatenated into one vpm_particles_t

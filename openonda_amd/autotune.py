@@ -105,6 +105,38 @@ def tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mode="uni
     return best, report
 
 
+def device_tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mode="uniform",
+                reps=2):
+    """Timing-only tuning inside the library (``vpm_autotune``): one sort, every
+    tile configuration timed with HIP events.  Returns (best index, ms list)."""
+    import ctypes as C
+
+    from . import _lib as L
+
+    P.require_gpu()
+    res = P.forward(pos, alpha, sigma, nu, outputs=outputs, precision="f32c",
+                    sigma_mode=sigma_mode)  # output buffers of the right shape
+    p = P._particles(pos, alpha, sigma, nu if "pse" in outputs else None)
+    mask = 0
+    for k in outputs:
+        mask |= P.OUTPUTS[k]
+    opts = L.ForwardOpts(mask, L.PREC_F32C,
+                         L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL, 0, -1)
+    vec = [k for k in ("vel", "stretch", "pse", "vort") if k in res]
+    ld = P._ld(res[vec[0]]) if vec else max(p.n, 1)
+    fo = L.ForwardOut(ld, *[res[k].data_ptr() if k in res else None
+                            for k in ("vel", "stretch", "pse", "vort", "enstrophy")])
+    lib = L.load()
+    nb = lib.vpm_autotune_workspace_bytes(p.n, C.byref(opts))
+    ws = P.workspace(pos.device, nb)
+    count = lib.vpm_tile_config_count()
+    ms = (C.c_float * count)()
+    best = C.c_int(-1)
+    L.call("vpm_autotune", C.byref(p), C.byref(opts), C.byref(fo), ws.data_ptr(), ws.numel(),
+           P._stream(pos), int(reps), C.byref(best), ms)
+    return best.value, [float(v) for v in ms]
+
+
 def main(argv=None):
     """python -m openonda_amd.autotune --n 1048576 --outputs vel,stretch [--cache PATH]"""
     import argparse

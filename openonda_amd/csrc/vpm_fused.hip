@@ -1274,6 +1274,67 @@ int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forwa
   return vpm_forward_run(p, o, out, workspace, ws_bytes, stream);
 }
 
+size_t vpm_autotune_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts) {
+  if (!opts) return 0;
+  vpm_forward_opts_t o = *opts;
+  size_t most = 0;
+  for (int c = 0; c < kNumCfgs; ++c) {
+    o.tile_config = c;
+    WsLayout L{};
+    if (layout(n, &o, &L) != VPM_OK) return 0;
+    most = std::max(most, L.total);
+  }
+  return most;
+}
+
+int vpm_autotune(const vpm_particles_t* p, const vpm_forward_opts_t* opts, vpm_forward_out_t* out,
+                 void* workspace, size_t ws_bytes, void* stream, int reps, int* best,
+                 float* ms) {
+  clear_error();
+  if (!opts || !best) return fail(VPM_ERR_ARG, "vpm_autotune: null opts/best");
+  if (reps < 1) return fail(VPM_ERR_ARG, "vpm_autotune: reps < 1");
+  if (opts->precision != VPM_PREC_F32C)
+    return fail(VPM_ERR_ARG, "vpm_autotune: tile configurations apply to VPM_PREC_F32C");
+  if (ws_bytes < vpm_autotune_workspace_bytes(p ? p->n : 0, opts))
+    return fail(VPM_ERR_WORKSPACE, "vpm_autotune: workspace < vpm_autotune_workspace_bytes");
+  vpm_forward_opts_t o = *opts;
+  o.tile_config = -1;
+  int rc = vpm_forward_prepare(p, &o, workspace, ws_bytes, stream);  // sorts once
+  if (rc) return rc;
+  hipStream_t st = as_stream(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VPM_HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(VPM_ERR_LAUNCH, "vpm_autotune: hipEventCreate failed");
+  }
+  float best_ms = 0.0f;
+  *best = -1;
+  for (int c = 0; c < kNumCfgs && rc == VPM_OK; ++c) {
+    o.tile_config = c;
+    rc = vpm_forward_run(p, &o, out, workspace, ws_bytes, stream);  // warm
+    if (rc) break;
+    if (hipEventRecord(e0, st) != hipSuccess) rc = VPM_ERR_LAUNCH;
+    for (int k = 0; k < reps && rc == VPM_OK; ++k)
+      rc = vpm_forward_run(p, &o, out, workspace, ws_bytes, stream);
+    float t = 0.0f;
+    if (rc == VPM_OK && (hipEventRecord(e1, st) != hipSuccess ||
+                         hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = fail(VPM_ERR_LAUNCH, "vpm_autotune: event timing failed");
+    if (rc) break;
+    t /= static_cast<float>(reps);
+    if (ms) ms[c] = t;
+    if (*best < 0 || t < best_ms) {
+      *best = c;
+      best_ms = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 size_t vpm_batch_workspace_bytes(int64_t n_total, int nsets, int n_items) {
   BatchLayout L{};
   if (batch_layout(n_total, nsets, n_items, &L) != VPM_OK) return 0;

@@ -248,3 +248,15 @@ def test_pse_accuracy_on_nearly_uniform_strengths(gpu):
     res = P.forward(*to_dev(x, a, s, nu), outputs=("vel", "stretch", "pse"), precision="f32c",
                     sigma_mode="uniform")
     assert rel_l2(P.from_soa(res["pse"]), ref) <= F32C_TOL
+
+
+def test_library_autotune_entry_point(gpu):
+    """vpm_autotune times every configuration in one call and names the fastest."""
+    from openonda_amd import autotune
+
+    x, a, s, nu = FM.random_cloud(16384, seed=12)
+    d = to_dev(x, a, s, nu)
+    print("[vpm_autotune] n=16384", flush=True)
+    best, ms = autotune.device_tune(d[0], d[1], d[2], outputs=("vel", "stretch"), reps=1)
+    assert len(ms) == len(P.tile_configs()) and all(t > 0 for t in ms)
+    assert 0 <= best < len(ms) and ms[best] == min(ms)
