@@ -142,7 +142,8 @@ def main(argv=None):
     # calibrated FETCH_SIZE + WRITE_SIZE runs summarised under profiles/
     from openonda_amd import profiling
 
-    cfg = P.tile_configs()[args.tile_config if args.tile_config >= 0 else 0]
+    cfg = P.tile_configs()[args.tile_config if args.tile_config >= 0
+                           else P.tile_config_default(outputs)]
     kernel_name = "fused_kernel<float, 3u, true, {}, {}, {}, {}, {}>".format(*cfg)
     traffic = profiling.load_traffic(traffic_workload(n), kernel_name)
 

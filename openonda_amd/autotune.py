@@ -138,7 +138,8 @@ def device_tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mo
 
 
 def main(argv=None):
-    """python -m openonda_amd.autotune --n 1048576 --outputs vel,stretch [--cache PATH]"""
+    """python -m openonda_amd.autotune --n 1048576 --outputs vel,stretch [--layout cloud|ring]
+    [--cache PATH]"""
     import argparse
 
     import numpy as np
@@ -150,9 +151,15 @@ def main(argv=None):
     ap.add_argument("--outputs", default="vel,stretch")
     ap.add_argument("--cache", default=None)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--layout", choices=["cloud", "ring"], default="cloud",
+                    help="random cloud, or a torus ring of n/64 stations")
     a = ap.parse_args(argv)
     outs = tuple(a.outputs.split(","))
-    x, al, s, nu = FM.random_cloud(a.n)
+    if a.layout == "ring":
+        x, al, s, nu = FM.torus_ring(max(1, a.n // 64))
+        a.n = len(x)
+    else:
+        x, al, s, nu = FM.random_cloud(a.n)
     dev = torch.device("cuda", 0)
     best, report = tune(P.to_soa(x, dev), P.to_soa(al, dev), torch.from_numpy(s).to(dev),
                         torch.from_numpy(np.full(a.n, 1e-3)).to(dev) if "pse" in outs else None,

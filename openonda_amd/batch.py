@@ -57,7 +57,9 @@ class BatchedSystem:
         self.precision = precision
         self.pse = pse
         self.les = les
-        self.tile_config = int(tile_config)
+        # one tile shape for every stage: the work items count its target blocks
+        self.tile_config = int(tile_config) if tile_config >= 0 else P.tile_config_default(
+            ("vel", "stretch", "pse") if pse else ("vel", "stretch"))
         sizes = [len(s[0]) for s in sets]
         for k, s in enumerate(sets):
             sig = np.asarray(s[2])
@@ -84,7 +86,7 @@ class BatchedSystem:
             tpi = 256
         else:
             cfgs = P.tile_configs()
-            b, r = cfgs[self.tile_config if self.tile_config >= 0 else 0][:2]
+            b, r = cfgs[self.tile_config][:2]
             tpi = b * r
         item_set, item_block, first, count = work_list(self.sizes, tpi)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(d)

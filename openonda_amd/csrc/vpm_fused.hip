@@ -642,15 +642,30 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(FusedArgs a, int nspl
 struct TileCfg {
   int block, r, tile, sub, comp;
 };
-// Index 0 is the FP32 default (the autotuner's winner for velocity +
-// stretching at N = 2^20: one target per lane, 64-source FP32 partials into a
-// Kahan pair per tile).  Configs 1.. exist for the hot output sets
-// (VEL|STRETCH, VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
+// Index 0 is the FP32 default without PSE (the autotuner's winner for
+// velocity + stretching at N = 2^20: one target per lane, 64-source FP32
+// partials into a Kahan pair per tile); index 1 the default with PSE (two
+// targets per lane, whose shared loads pay for the longer pair).  Both exist
+// for every output set; configs 2.. for the hot output sets (VEL|STRETCH,
+// VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
 constexpr TileCfg kCfgs[] = {
-    {256, 1, 256, 64, 0},  {256, 2, 256, 32, 1}, {256, 2, 256, 32, 0}, {256, 1, 256, 64, 1},
-    {256, 1, 256, 128, 0}, {256, 1, 512, 64, 0}, {512, 1, 512, 64, 0}, {128, 1, 256, 64, 0},
-    {256, 2, 512, 64, 0},  {256, 4, 256, 32, 0},
+    {512, 1, 512, 64, 0},  {256, 2, 256, 32, 1},  {256, 1, 256, 64, 0}, {256, 1, 256, 64, 1},
+    {256, 1, 512, 64, 0},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
+    {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},
 };
+
+constexpr int default_config(unsigned kernel_mask) {
+  return (kernel_mask & VPM_OUT_PSE) ? 1 : 0;
+}
+
+constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
+
+// The configuration the single-set launch really uses for a request.
+constexpr int effective_config(unsigned kernel_mask, bool unif, int requested) {
+  return requested < 0 ? default_config(kernel_mask)
+         : (requested <= 1 || (unif && hot_mask(kernel_mask))) ? requested
+                                                                : default_config(kernel_mask);
+}
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
 
@@ -700,21 +715,19 @@ int launch_f32(const FusedArgs& a, int cfg, hipStream_t st, int* nb) {
   case I:                                                                                    \
     return launch_cfg<float, OUT, UNIF, kCfgs[I].block, kCfgs[I].r, kCfgs[I].tile,          \
                       kCfgs[I].sub, kCfgs[I].comp>(a, st, nb);
-    VPM_CFG(0)
+    VPM_CFG(0) VPM_CFG(1)
     default:
       break;
   }
-  if constexpr (UNIF && (OUT == 3u || OUT == 7u || OUT == 27u)) {
+  if constexpr (UNIF && hot_mask(OUT)) {
     switch (cfg) {
-      VPM_CFG(1) VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7)
-      VPM_CFG(8) VPM_CFG(9)
+      VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7) VPM_CFG(8) VPM_CFG(9)
       default:
         break;
     }
 #undef VPM_CFG
   }
-  return launch_cfg<float, OUT, UNIF, kCfgs[0].block, kCfgs[0].r, kCfgs[0].tile, kCfgs[0].sub,
-                    kCfgs[0].comp>(a, st, nb);
+  return VPM_ERR_ARG;  // callers pass effective_config()
 }
 
 template <unsigned OUT>
@@ -773,7 +786,8 @@ size_t split_capacity(int64_t n, const vpm_forward_opts_t* o) {
   if (!o || n <= 0) return 0;
   int br = kCfgF64.block * kCfgF64.r, tile = kCfgF64.tile;
   if (o->precision == VPM_PREC_F32C) {
-    const int c = (o->tile_config < 0 || o->tile_config >= kNumCfgs) ? 0 : o->tile_config;
+    const int c = effective_config(kernel_mask(o->outputs), o->sigma_mode == VPM_SIGMA_UNIFORM,
+                                   o->tile_config < kNumCfgs ? o->tile_config : -1);
     br = kCfgs[c].block * kCfgs[c].r;
     tile = kCfgs[c].tile;
   }
@@ -1108,6 +1122,10 @@ extern "C" {
 
 int vpm_tile_config_count(void) { return kNumCfgs; }
 
+int vpm_tile_config_default(uint32_t outputs) {
+  return default_config(kernel_mask(outputs == 0 ? VPM_OUT_VEL : outputs));
+}
+
 int vpm_tile_config_info(int index, int* block, int* r, int* tile, int* sub, int* comp) {
   clear_error();
   if (index < 0 || index >= kNumCfgs) return fail(VPM_ERR_ARG, "tile config index out of range");
@@ -1254,7 +1272,7 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   a.max_splits = o->max_splits;
   a.g64_mode = o->g64_mode;
   int nblocks = 0;
-  const int cfg = o->tile_config < 0 ? 0 : o->tile_config;
+  const int cfg = effective_config(mask, o->sigma_mode == VPM_SIGMA_UNIFORM, o->tile_config);
   if ((rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st,
                      &nblocks)))
     return rc;
@@ -1431,7 +1449,8 @@ int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm
   b.item_block = d->item_block;
   b.n_items = d->n_items;
   b.counter = reinterpret_cast<int*>(ws + L.counter);
-  if ((rc = dispatch_batch(b, batch_mask(want), o->precision, o->tile_config < 0 ? 0 : o->tile_config,
+  if ((rc = dispatch_batch(b, batch_mask(want), o->precision,
+                           o->tile_config < 0 ? default_config(batch_mask(want)) : o->tile_config,
                            d->grid, st)))
     return rc;
   if ((want & VPM_OUT_ENSTROPHY) && d->nsets > 0) {

@@ -279,6 +279,16 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
     return res
 
 
+def tile_config_default(outputs=("vel", "stretch")):
+    """Index of the tile configuration used for ``tile_config=-1``."""
+    if isinstance(outputs, str):
+        outputs = (outputs,)
+    mask = 0
+    for k in outputs:
+        mask |= OUTPUTS[k]
+    return int(L.load().vpm_tile_config_default(mask))
+
+
 def tile_configs():
     """List of (block, targets_per_lane, tile, subtile, comp) per config index."""
     lib = L.load()
