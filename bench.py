@@ -290,14 +290,17 @@ def main(argv=None):
         # per-set accuracy of one FP32-compensated evaluation against the plain FP64 kernels
         bs.evaluate(bs.x, bs.a, 3, True, bs.stream.cuda_stream)
         bs.synchronize()
-        errs = []
+        errs, per_set = [], []
         for k in range(bs.nsets):
             sl = bs.set_slice(k)
             xs, as_, ss = bs.x[:, sl], bs.a[:, sl], bs.sigma[sl]
             u = P.selfinduced_velocity(xs, as_, ss)
             sg = P.strength_gradients(xs, as_, ss)
-            errs.append(max(float((bs.vel[:, sl] - u).norm() / u.norm()),
-                            float((bs.str_[:, sl] - sg).norm() / sg.norm())))
+            ev = float((bs.vel[:, sl] - u).norm() / u.norm())
+            es = float((bs.str_[:, sl] - sg).norm() / sg.norm())
+            errs.append(max(ev, es))
+            per_set.append({"set": k, "n": int(bs.sizes[k]), "kind": "ring" if k % 2 == 0 else "cloud",
+                            "vel": ev, "stretch": es})
         pairs_step = 4 * bs.pairs_per_evaluation()
         result["batch"] = {
             "workload": f"{bs.nsets} ring/cloud sets, N_i log-uniform in [2048, 131072], "
@@ -305,7 +308,8 @@ def main(argv=None):
                         f"one persistent launch per stage, HIP-graph replay",
             "steps": args.batch_steps, "ms_per_step": bms / args.batch_steps,
             "pairs_per_s": pairs_step * args.batch_steps / (bms * 1e-3),
-            "rel_l2_max_vs_fp64": max(errs), "rel_l2_median_vs_fp64": float(np.median(errs))}
+            "rel_l2_max_vs_fp64": max(errs), "rel_l2_median_vs_fp64": float(np.median(errs)),
+            "worst_sets": sorted(per_set, key=lambda r: -max(r["vel"], r["stretch"]))[:3]}
         log(f"[bench] batch: {bms / args.batch_steps:.1f} ms/step, "
             f"{result['batch']['pairs_per_s']:.3e} pairs/s, max rel-L2 {max(errs):.2e}")
 
