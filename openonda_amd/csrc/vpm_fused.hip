@@ -643,14 +643,15 @@ struct TileCfg {
   int block, r, tile, sub, comp;
 };
 // Index 0 is the FP32 default without PSE (the autotuner's winner for
-// velocity + stretching at N = 2^20: one target per lane, 64-source FP32
-// partials into a Kahan pair per tile); index 1 the default with PSE (two
-// targets per lane, whose shared loads pay for the longer pair).  Both exist
-// for every output set; configs 2.. for the hot output sets (VEL|STRETCH,
-// VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
+// velocity + stretching on a random cloud of 2^20: 512 lanes x one target,
+// 64-source FP32 partials into a Kahan pair per tile); index 1 the default
+// with PSE (the winner for velocity + stretching + PSE on a ring of 131072:
+// 256 lanes, 512-source tiles).  Both exist for every output set; configs
+// 2.. for the hot output sets (VEL|STRETCH, VEL|STRETCH|PSE,
+// VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
 constexpr TileCfg kCfgs[] = {
-    {512, 1, 512, 64, 0},  {256, 2, 256, 32, 1},  {256, 1, 256, 64, 0}, {256, 1, 256, 64, 1},
-    {256, 1, 512, 64, 0},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
+    {512, 1, 512, 64, 0},  {256, 1, 512, 64, 0},  {256, 2, 256, 32, 1}, {256, 1, 256, 64, 0},
+    {256, 1, 256, 64, 1},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
     {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},
 };
 

@@ -75,13 +75,13 @@ def test_argument_errors_return_status_without_launch():
 
 
 def test_tile_config_defaults():
-    """tile_config = -1 picks one shape per output set: one target per lane
-    without PSE, two with PSE (both exist for every output set)."""
+    """tile_config = -1 picks one shape per output set (both shapes exist for
+    every output set): 512 lanes without PSE, 256 lanes with PSE."""
     lib = L.load()
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 1
     assert lib.vpm_tile_config_default(31) == 1
     vals = [ctypes.c_int() for _ in range(5)]
-    for idx, want_r in ((0, 1), (1, 2)):
+    for idx, want_block in ((0, 512), (1, 256)):
         assert L.call("vpm_tile_config_info", idx, *[ctypes.byref(v) for v in vals]) == 0
-        assert vals[1].value == want_r
+        assert vals[0].value == want_block
