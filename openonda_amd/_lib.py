@@ -126,6 +126,7 @@ SIGNATURES = {
     "vpm_graph_destroy": (C.c_int, [_VP]),
     "vpm_tile_config_count": (C.c_int, []),
     "vpm_tile_config_default": (C.c_int, [C.c_uint32]),
+    "vpm_batch_tile_config_default": (C.c_int, [C.c_uint32]),
     "vpm_tile_config_info": (C.c_int, [C.c_int] + [_P(C.c_int)] * 5),
     "vpm_autotune_workspace_bytes": (C.c_size_t, [C.c_int64, _P(ForwardOpts)]),
     "vpm_autotune": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP, C.c_size_t,

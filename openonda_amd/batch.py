@@ -58,8 +58,9 @@ class BatchedSystem:
         self.pse = pse
         self.les = les
         # one tile shape for every stage: the work items count its target blocks
-        self.tile_config = int(tile_config) if tile_config >= 0 else P.tile_config_default(
-            ("vel", "stretch", "pse") if pse else ("vel", "stretch"))
+        stage = L.OUT_VEL | L.OUT_STRETCH | (L.OUT_PSE if pse else 0)
+        self.tile_config = (int(tile_config) if tile_config >= 0
+                            else int(L.load().vpm_batch_tile_config_default(stage)))
         sizes = [len(s[0]) for s in sets]
         for k, s in enumerate(sets):
             sig = np.asarray(s[2])

@@ -81,6 +81,8 @@ def test_tile_config_defaults():
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 1
     assert lib.vpm_tile_config_default(31) == 1
+    assert lib.vpm_batch_tile_config_default(31) == 2
+    assert lib.vpm_batch_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     vals = [ctypes.c_int() for _ in range(5)]
     for idx, want_block in ((0, 512), (1, 256)):
         assert L.call("vpm_tile_config_info", idx, *[ctypes.byref(v) for v in vals]) == 0
