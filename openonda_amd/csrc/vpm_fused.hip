@@ -68,10 +68,26 @@ struct Arith<float> {
   static __device__ inline float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
   static __device__ inline float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
+// FP64: the hardware estimates refined by two Newton steps (each squares the
+// relative error of the estimate) instead of the IEEE sqrt + division
+// sequences; the FP64 parity tests hold the pass to 1e-12 of the oracle.
+// Arguments are finite and >= 1 (A) or > 0 (sigma sums): no special cases.
 template <>
 struct Arith<double> {
-  static __device__ inline double rsq(double a) { return 1.0 / sqrt(a); }
-  static __device__ inline double rcp(double a) { return 1.0 / a; }
+  static __device__ inline double rsq(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    double e = ::fma(-a * y, y, 1.0);  // 1 - a y^2
+    y = ::fma(0.5 * y, e, y);
+    e = ::fma(-a * y, y, 1.0);
+    return ::fma(0.5 * y, e, y);
+  }
+  static __device__ inline double rcp(double a) {
+    double y = __builtin_amdgcn_rcp(a);
+    double e = ::fma(-a, y, 1.0);      // 1 - a y
+    y = ::fma(y, e, y);
+    e = ::fma(-a, y, 1.0);
+    return ::fma(y, e, y);
+  }
   static __device__ inline double fma(double a, double b, double c) { return ::fma(a, b, c); }
 };
 
