@@ -229,9 +229,20 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   if constexpr (S::vel || S::str) {
     Real g = M::fma(k.c15, t5, t3);
     if constexpr (!UNIF) g *= inv2 * inv;
-    const Real cx = dy * s.az - dz * s.ay;
-    const Real cy = dz * s.ax - dx * s.az;
-    const Real cz = dx * s.ay - dy * s.ax;
+    Real cx = dy * s.az - dz * s.ay;
+    Real cy = dz * s.ax - dx * s.az;
+    Real cz = dx * s.ay - dy * s.ax;
+    // Coherent waves (G64) of an FP32 pass with stretching: c = r x alpha_j
+    // and e = alpha_i . c also take the strengths' low parts.  Along a vortex
+    // tube alpha_i, r and alpha_j are nearly coplanar and e is a small
+    // triple product, so the rounding of the FP32 strengths (~eps |alpha|)
+    // would dominate it (thin rings: rel-L2 4e-6 -> 9e-7 in emulation).
+    constexpr bool kLoCross = G64 && S::str && std::is_same<Real, float>::value;
+    if constexpr (kLoCross) {
+      cx = M::fma(dy, slo[2], M::fma(-dz, slo[1], cx));
+      cy = M::fma(dz, slo[0], M::fma(-dx, slo[2], cy));
+      cz = M::fma(dx, slo[1], M::fma(-dy, slo[0], cz));
+    }
     if constexpr (S::vel) {
       acc[kU + 0] += g * cx;
       acc[kU + 1] += g * cy;
@@ -248,7 +259,8 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
         acc[kG + 1] += g * s.ay;
         acc[kG + 2] += g * s.az;
       }
-      const Real e = t.ax * cx + t.ay * cy + t.az * cz;
+      Real e = t.ax * cx + t.ay * cy + t.az * cz;
+      if constexpr (kLoCross) e += t.lx * cx + t.ly * cy + t.lz * cz;
       Real f2 = t5 * M::fma(k.c75, t2, k.c3);
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
       const Real w = f2 * e;
@@ -340,7 +352,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
   constexpr bool kFold32 = std::is_same<Real, float>::value;
-  constexpr bool kLo = S::pse && std::is_same<Real, float>::value;
+  // low parts alpha - fp32(alpha) of the strengths (PSE differences; the
+  // stretching's triple product in coherent waves)
+  constexpr bool kLo = (S::pse || S::str) && std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
