@@ -689,12 +689,10 @@ constexpr int default_config(unsigned kernel_mask) {
   return (kernel_mask & VPM_OUT_PSE) ? 1 : 0;
 }
 
-// The batched launch with PSE keeps two targets per lane: its stages carry
-// LES vorticity and enstrophy too, and measured 959 ms per RK4 step with
-// config 1 against 918 ms with config 2 (96 sets, 3.3e6 particles).
-constexpr int default_batch_config(unsigned batch_mask) {
-  return (batch_mask & VPM_OUT_PSE) ? 2 : 0;
-}
+// The batched launch uses the same shapes (96 sets, 3.3e6 particles, RK4 with
+// PSE and LES: 1068 ms per step with config 1, 1070-1092 ms with the other
+// one-target shapes, 1081-1205 ms with two targets per lane).
+constexpr int default_batch_config(unsigned batch_mask) { return default_config(batch_mask); }
 
 constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
 

@@ -81,7 +81,7 @@ def test_tile_config_defaults():
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 1
     assert lib.vpm_tile_config_default(31) == 1
-    assert lib.vpm_batch_tile_config_default(31) == 2
+    assert lib.vpm_batch_tile_config_default(31) == 1
     assert lib.vpm_batch_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     vals = [ctypes.c_int() for _ in range(5)]
     for idx, want_block in ((0, 512), (1, 256)):
