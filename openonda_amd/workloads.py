@@ -204,14 +204,16 @@ def examples(dev, steps=10, names=("E1", "E2", "E3", "E4", "E5"), precision="pla
                "kinetic_energy_drift": ps.get_total_kinetic_energy() / e0 - 1.0}
         _log(f"[examples] {row}")
         rows.append(row)
-    return {"workload": f"reference examples E1-E5 at script resolution, {steps} steps each",
-            "rows": rows}
+    return {"workload": f"reference examples E1-E5 at script resolution, {steps} steps each, "
+                        f"precision {precision}", "rows": rows}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("name", choices=["ring", "leapfrog", "sweep", "rollout", "examples"])
     ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--precision", choices=["plain", "f64", "f32c"], default="plain",
+                    help="ParticleSystem arithmetic for the examples workload")
     a = ap.parse_args(argv)
     P.require_gpu()
     dev = torch.device("cuda", 0)
@@ -222,7 +224,7 @@ def main(argv=None):
     elif a.name == "sweep":
         r = sweep(dev)
     elif a.name == "examples":
-        r = examples(dev, steps=a.steps or 10)
+        r = examples(dev, steps=a.steps or 10, precision=a.precision)
     else:
         r = rollout(dev, steps=a.steps or 50)
     print(json.dumps(r), flush=True)
