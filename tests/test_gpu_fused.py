@@ -260,3 +260,18 @@ def test_library_autotune_entry_point(gpu):
     best, ms = autotune.device_tune(d[0], d[1], d[2], outputs=("vel", "stretch"), reps=1)
     assert len(ms) == len(P.tile_configs()) and all(t > 0 for t in ms)
     assert 0 <= best < len(ms) and ms[best] == min(ms)
+
+
+def test_thin_ring_stretching_with_low_parts(gpu):
+    """A thin ring (1978 stations, core 3h ~ 0.01, sigma = 1.3h) has
+    |S| ~ 2e-3 |alpha||u|: the stretching is a near-cancellation.  Coherent
+    waves feed the strengths' low parts into the triple product; without
+    them the FP32 pass lands at ~4e-6."""
+    x, a, s, nu = FM.torus_ring(1978, h=2 * np.pi / 1978)
+    print(f"[thin-ring] n={len(x)}", flush=True)
+    d = to_dev(x, a, s, nu)
+    res = P.forward(*d, outputs=("vel", "stretch"), precision="f32c", sigma_mode="uniform")
+    u = P.selfinduced_velocity(d[0], d[1], d[2])
+    st = P.strength_gradients(d[0], d[1], d[2])
+    assert float((res["vel"] - u).norm() / u.norm()) <= 1e-6
+    assert float((res["stretch"] - st).norm() / st.norm()) <= 1.5e-6
