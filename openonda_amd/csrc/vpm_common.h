@@ -129,6 +129,17 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+__device__ inline float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 // Deterministic block sum (fixed shuffle tree + fixed LDS order).  All
 // threads of the block must call it; the result is valid in thread 0.
 template <int BLOCK>

@@ -91,8 +91,9 @@ struct Arith<double> {
   static __device__ inline double fma(double a, double b, double c) { return ::fma(a, b, c); }
 };
 
-// LDS source record.  Uniform sigma needs position + strength; general sigma
-// adds sigma_j/2, 1/sigma_j^2 and 1/sigma_j^3.
+// LDS source record.  Uniform sigma needs position + strength (p0, p1: x and
+// y of the moment a_j x x_j of the far-field form); general sigma adds
+// sigma_j/2, 1/sigma_j^2 and 1/sigma_j^3.
 template <typename Real, bool UNIF>
 struct alignas(16) SrcR;
 template <typename Real>
@@ -110,8 +111,9 @@ struct Tgt {
   Real x, y, z, ax, ay, az, hs, lx, ly, lz;
 };
 
-// Accumulator slots.
-enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kW = 12, kE = 15, kNAcc = 18 };
+// Accumulator slots.  kH, the moment sum of the far-field form, shares the
+// PSE slots: that form runs only for output sets without PSE.
+enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kH = 9, kW = 12, kE = 15, kNAcc = 18 };
 
 template <unsigned OUT>
 struct Slots {
@@ -309,6 +311,52 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   }
 }
 
+// Far-field moment form of velocity + stretching (FP32, uniform radius,
+// waves whose strengths are unrelated).  With d = x_i - x_j and coordinates
+// relative to the workgroup centre,
+//   sum_j g d x a_j  = x_i x G + sum_j g (a_j x x_j),   G = sum_j g a_j,
+//   a_i . (d x a_j)  = a_j . (a_i x x_i) + a_i . (a_j x x_j),
+// so with G already summed for the stretching and the source moments
+// m_j = a_j x x_j staged with the tile, a pair costs 29 VALU instructions
+// instead of 32 (the cross product d x a_j and its three multiply-adds
+// into u go).  x_i x G + H is a cancellation of relative size |x_i| / |d|:
+// the form is used only for source tiles whose bounding box keeps a gap of
+// more than kFarGap radii from the workgroup's targets, where it stays
+// small (random cloud of 2^20 in FP32 emulation: rel-L2 unchanged at
+// 3.1e-7 / 6.2e-7 with 97% of the pairs in the far form; using it for
+// every pair gives 1.2e-6 / 8.7e-7).  p = a_i x x_i per target.
+constexpr double kFarGap = 4.0;
+
+template <typename Real, bool UNIF>
+__device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const Real* p,
+                                            const SrcR<Real, UNIF>& s, const float4& m,
+                                            const PairConst<Real>& k) {
+  using M = Arith<Real>;
+  const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+  const Real A = M::fma(dz, dz, M::fma(dy, dy, M::fma(dx, dx, Real(1))));
+  const Real t1 = M::rsq(A);
+  const Real t2 = t1 * t1;
+  const Real t3 = t2 * t1;
+  const Real t5 = t3 * t2;
+  const Real g = M::fma(k.c15, t5, t3);
+  acc[kG + 0] = M::fma(g, s.ax, acc[kG + 0]);
+  acc[kG + 1] = M::fma(g, s.ay, acc[kG + 1]);
+  acc[kG + 2] = M::fma(g, s.az, acc[kG + 2]);
+  acc[kH + 0] = M::fma(g, m.x, acc[kH + 0]);
+  acc[kH + 1] = M::fma(g, m.y, acc[kH + 1]);
+  acc[kH + 2] = M::fma(g, m.z, acc[kH + 2]);
+  Real e = s.ax * p[0];
+  e = M::fma(s.ay, p[1], e);
+  e = M::fma(s.az, p[2], e);
+  e = M::fma(t.ax, m.x, e);
+  e = M::fma(t.ay, m.y, e);
+  e = M::fma(t.az, m.z, e);
+  const Real w = t5 * M::fma(k.c75, t2, k.c3) * e;
+  acc[kSd + 0] = M::fma(w, dx, acc[kSd + 0]);
+  acc[kSd + 1] = M::fma(w, dy, acc[kSd + 1]);
+  acc[kSd + 2] = M::fma(w, dz, acc[kSd + 2]);
+}
+
 // Coherence of 64 consecutive sorted targets (one lane row of a wave),
 // |sum alpha_i| / sum |alpha_i|: for unrelated strengths it is ~0.14 and
 // stayed below 0.35 over the 16384 rows of a random cloud of 2^20 particles;
@@ -355,13 +403,28 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // low parts alpha - fp32(alpha) of the strengths (PSE differences; the
   // stretching's triple product in coherent waves)
   constexpr bool kLo = (S::pse || S::str) && std::is_same<Real, float>::value;
+  // Velocity/stretching output sets of an FP32 pass carry two loops: FP64
+  // per-pair G for coherent waves, FP32 partials of G otherwise (with PSE,
+  // vorticity or enstrophy the second loop costs occupancy, and G is summed
+  // in FP64).  With velocity and a uniform radius the FP32-G waves also use
+  // the far-field moment form (pair_moment) on distant tiles.
+  constexpr bool kBothG = S::str && !S::pse && !S::vort && !S::ens;
+  constexpr bool kMoment = kBothG && S::vel && UNIF && std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
+  __shared__ float lds_mz[kMoment ? TILE : 1];  // a_j x x_j: (p0, p1) of the record, z here
+  __shared__ double lds_x64[kMoment ? BLOCK * R : 1][3];  // the targets' x_i in FP64
   __shared__ double red[6][BLOCK / 64];
   __shared__ double centre[3];
+  __shared__ double half_ext[3];  // the targets' bounding box, scaled: [-half_ext, half_ext]
 
   const int tid = threadIdx.x;
+  // Uniform sigma: every record holds the same radius; offsets are scaled by
+  // 1/sigma so that r^2 below is rho^2.  The vorticity mask r^2 > 1e-8
+  // (ParticlesPhysicsCuda.py:332) is rescaled alike.
+  const double sigma0 = UNIF ? src[0].s : 1.0;
+  const double scale = UNIF ? 1.0 / sigma0 : 1.0;
 
   // ---- targets and the block centre ------------------------------------
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -398,15 +461,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
         hi = fmax(hi, red[3 + tid][w]);
       }
       centre[tid] = 0.5 * (lo + hi);
+      half_ext[tid] = 0.5 * (hi - lo) * scale;
     }
     __syncthreads();
   }
   const double cx = centre[0], cy = centre[1], cz = centre[2];
-  // Uniform sigma: every record holds the same radius; offsets are scaled by
-  // 1/sigma so that r^2 below is rho^2.  The vorticity mask r^2 > 1e-8
-  // (ParticlesPhysicsCuda.py:332) is rescaled alike.
-  const double sigma0 = UNIF ? src[0].s : 1.0;
-  const double scale = UNIF ? 1.0 / sigma0 : 1.0;
   const double eps_r2_d = kEpsPair * scale * scale;
 
   Tgt<Real> t[R];
@@ -426,11 +485,23 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       t[r].lz = static_cast<Real>(tg[r].az - static_cast<double>(t[r].az));
     }
   }
+  Real pm[R][3];  // a_i x x_i of the moment form
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    pm[r][0] = pm[r][1] = pm[r][2] = Real(0);
+    if constexpr (kMoment) {
+      const double x = (tg[r].x - cx) * scale, y = (tg[r].y - cy) * scale,
+                   z = (tg[r].z - cz) * scale;
+      lds_x64[r * BLOCK + tid][0] = x;
+      lds_x64[r * BLOCK + tid][1] = y;
+      lds_x64[r * BLOCK + tid][2] = z;
+      pm[r][0] = static_cast<Real>(tg[r].ay * z - tg[r].az * y);
+      pm[r][1] = static_cast<Real>(tg[r].az * x - tg[r].ax * z);
+      pm[r][2] = static_cast<Real>(tg[r].ax * y - tg[r].ay * x);
+    }
+  }
   const Real eps_r2 = static_cast<Real>(eps_r2_d);
-  // Wave-uniform choice of the G accumulation (FP32 passes).  Only the
-  // velocity/stretching output sets carry both loops: with PSE, vorticity or
-  // enstrophy the second loop costs occupancy, and G is summed in FP64.
-  constexpr bool kBothG = S::str && !S::pse && !S::vort && !S::ens;
+  // Wave-uniform choice of the G accumulation (FP32 passes).
   bool g64 = S::str && std::is_same<Real, float>::value;
   if constexpr (kBothG && std::is_same<Real, float>::value) {
     g64 = a.g64_mode == VPM_G64_ALWAYS ||
@@ -459,130 +530,210 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   if constexpr (std::is_same<Real, float>::value)  // keep them in VGPRs
     asm volatile("" : "+v"(kc.c15), "+v"(kc.c75), "+v"(kc.c22), "+v"(kc.c3));
 
-  // Sources per unrolled step: deep unrolling lets the compiler prefetch LDS
-  // records for 8 sources, which the heaviest output sets pay for in occupancy.
-  constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
-
   // ---- sweep the source tiles ------------------------------------------
-  for (int t0 = lo; t0 < hi; t0 += TILE) {
-    __syncthreads();
+  // One instantiation per G mode (G64: FP64 per-pair G, coherent waves), so
+  // that each loop keeps only its own state live.  The waves of one
+  // workgroup may take different modes and share the staged tile: every
+  // wave stages (and bounds) its share of every LDS array either mode reads.
+  auto sweep = [&](auto mode) {
+    constexpr bool G64 = decltype(mode)::value;
+    constexpr bool kFar = kMoment && !G64;
+    constexpr bool kStageLo = kLo;
+    constexpr bool kStageA64 = S::str;
+    // Sources per unrolled step: deep unrolling lets the compiler prefetch LDS
+    // records for 8 sources, which the heaviest output sets pay for in occupancy.
+    constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
+    for (int t0 = lo; t0 < hi; t0 += TILE) {
+      __syncthreads();
+      float bmn[3] = {INFINITY, INFINITY, INFINITY}, bmx[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-    for (int k = tid; k < TILE; k += BLOCK) {
-      const int j = t0 + k;
-      SrcR<Real, UNIF> s;
-      if (j < hi) {
-        const Src8 g = src[j];
-        s.x = static_cast<Real>((g.x - cx) * scale);
-        s.y = static_cast<Real>((g.y - cy) * scale);
-        s.z = static_cast<Real>((g.z - cz) * scale);
-        s.ax = static_cast<Real>(g.ax);
-        s.ay = static_cast<Real>(g.ay);
-        s.az = static_cast<Real>(g.az);
-        if constexpr (S::str) {
-          lds_a64[k][0] = g.ax;
-          lds_a64[k][1] = g.ay;
-          lds_a64[k][2] = g.az;
+      for (int k = tid; k < TILE; k += BLOCK) {
+        const int j = t0 + k;
+        SrcR<Real, UNIF> s;
+        if (j < hi) {
+          const Src8 g = src[j];
+          const double x = (g.x - cx) * scale, y = (g.y - cy) * scale, z = (g.z - cz) * scale;
+          s.x = static_cast<Real>(x);
+          s.y = static_cast<Real>(y);
+          s.z = static_cast<Real>(z);
+          s.ax = static_cast<Real>(g.ax);
+          s.ay = static_cast<Real>(g.ay);
+          s.az = static_cast<Real>(g.az);
+          if constexpr (kMoment) {
+            s.p0 = static_cast<Real>(g.ay * z - g.az * y);
+            s.p1 = static_cast<Real>(g.az * x - g.ax * z);
+            lds_mz[k] = static_cast<float>(g.ax * y - g.ay * x);
+            bmn[0] = fminf(bmn[0], s.x); bmx[0] = fmaxf(bmx[0], s.x);
+            bmn[1] = fminf(bmn[1], s.y); bmx[1] = fmaxf(bmx[1], s.y);
+            bmn[2] = fminf(bmn[2], s.z); bmx[2] = fmaxf(bmx[2], s.z);
+          }
+          if constexpr (kStageA64) {
+            lds_a64[k][0] = g.ax;
+            lds_a64[k][1] = g.ay;
+            lds_a64[k][2] = g.az;
+          }
+          if constexpr (kStageLo)
+            lds_lo[k] = make_float4(static_cast<float>(g.ax - static_cast<double>(s.ax)),
+                                    static_cast<float>(g.ay - static_cast<double>(s.ay)),
+                                    static_cast<float>(g.az - static_cast<double>(s.az)), 0.0f);
+          if constexpr (!UNIF) {
+            s.hs = static_cast<Real>(0.5 * g.s);
+            s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
+            s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
+          }
+        } else {  // padding: far away, zero strength -> contributes exactly 0
+          s.x = s.y = s.z = static_cast<Real>(1e18);
+          s.ax = s.ay = s.az = Real(0);
+          if constexpr (kStageA64) {
+            lds_a64[k][0] = 0.0;
+            lds_a64[k][1] = 0.0;
+            lds_a64[k][2] = 0.0;
+          }
+          if constexpr (kStageLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if constexpr (kMoment) {
+            s.p0 = s.p1 = Real(0);
+            lds_mz[k] = 0.0f;
+          }
+          if constexpr (!UNIF) {
+            s.hs = Real(0.5);
+            s.is2 = Real(1);
+            s.is3 = Real(1);
+          }
         }
-        if constexpr (kLo)
-          lds_lo[k] = make_float4(static_cast<float>(g.ax - static_cast<double>(s.ax)),
-                                  static_cast<float>(g.ay - static_cast<double>(s.ay)),
-                                  static_cast<float>(g.az - static_cast<double>(s.az)), 0.0f);
-        if constexpr (!UNIF) {
-          s.hs = static_cast<Real>(0.5 * g.s);
-          s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
-          s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
-        }
-      } else {  // padding: far away, zero strength -> contributes exactly 0
-        s.x = s.y = s.z = static_cast<Real>(1e18);
-        s.ax = s.ay = s.az = Real(0);
-        if constexpr (S::str) {
-          lds_a64[k][0] = 0.0;
-          lds_a64[k][1] = 0.0;
-          lds_a64[k][2] = 0.0;
-        }
-        if constexpr (kLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if constexpr (!UNIF) {
-          s.hs = Real(0.5);
-          s.is2 = Real(1);
-          s.is3 = Real(1);
+        lds[k] = s;
+      }
+      if constexpr (kMoment) {  // the tile's bounding box, per wave
+        const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float l = wave_min(bmn[c]), h = wave_max(bmx[c]);
+          if (lane == 0) {
+            red[c][wave] = static_cast<double>(l);
+            red[3 + c][wave] = static_cast<double>(h);
+          }
         }
       }
-      lds[k] = s;
-    }
-    __syncthreads();
+      __syncthreads();
+      // Moment form for this tile: its gap to the targets exceeds kFarGap.
+      bool far = false;
+      if constexpr (kFar) {
+        double gap2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double l = red[c][0], h = red[3 + c][0];
+#pragma unroll
+          for (int w = 1; w < BLOCK / 64; ++w) {
+            l = fmin(l, red[c][w]);
+            h = fmax(h, red[3 + c][w]);
+          }
+          const double gp = fmax(fmax(l - half_ext[c], -half_ext[c] - h), 0.0);
+          gap2 += gp * gp;
+        }
+        far = __builtin_amdgcn_readfirstlane(static_cast<int>(gap2 > kFarGap * kFarGap)) != 0;
+      }
 
-    for (int sb = 0; sb < TILE; sb += SUB) {
-      if (g64) {  // constant unless kBothG
+      for (int sb = 0; sb < TILE; sb += SUB) {
+        if constexpr (G64) {
 #pragma unroll kUnroll
-        for (int q = 0; q < SUB; ++q) {
-          const SrcR<Real, UNIF> s = lds[sb + q];
-          const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1], lds_a64[sb + q][2]};
-          Real slo[3] = {Real(0), Real(0), Real(0)};
-          if constexpr (kLo) {
-            const float4 l = lds_lo[sb + q];
-            slo[0] = l.x;
-            slo[1] = l.y;
-            slo[2] = l.z;
-          }
+          for (int q = 0; q < SUB; ++q) {
+            const SrcR<Real, UNIF> s = lds[sb + q];
+            const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1], lds_a64[sb + q][2]};
+            Real slo[3] = {Real(0), Real(0), Real(0)};
+            if constexpr (kStageLo) {
+              const float4 l = lds_lo[sb + q];
+              slo[0] = l.x;
+              slo[1] = l.y;
+              slo[2] = l.z;
+            }
 #pragma unroll
-          for (int r = 0; r < R; ++r)
-            pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2, kc);
-        }
-      } else {
+            for (int r = 0; r < R; ++r)
+              pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2,
+                                                 kc);
+          }
+        } else {
+          if (far) {  // constant false unless kFar
 #pragma unroll kUnroll
-        for (int q = 0; q < SUB; ++q) {
-          const SrcR<Real, UNIF> s = lds[sb + q];
-          Real slo[3] = {Real(0), Real(0), Real(0)};
-          if constexpr (kLo) {
-            const float4 l = lds_lo[sb + q];
-            slo[0] = l.x;
-            slo[1] = l.y;
-            slo[2] = l.z;
-          }
+            for (int q = 0; q < SUB; ++q) {
+              const SrcR<Real, UNIF> s = lds[sb + q];
+              const float4 m = make_float4(s.p0, s.p1, lds_mz[sb + q], 0.0f);
 #pragma unroll
-          for (int r = 0; r < R; ++r)
-            pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, slo, eps_r2,
-                                                kc);
-        }
-      }
-      if constexpr (kFold32) {
+              for (int r = 0; r < R; ++r) {
+                if constexpr (kFar) pair_moment<Real, UNIF>(part[r], t[r], pm[r], s, m, kc);
+              }
+            }
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
+            for (int r = 0; r < R; ++r) {  // u += x_i x G + H of the sub-tile, in FP64
+              if constexpr (kFar) {
+                const double X = lds_x64[r * BLOCK + tid][0], Y = lds_x64[r * BLOCK + tid][1],
+                             Z = lds_x64[r * BLOCK + tid][2];
+                const double gx = part[r][kG + 0], gy = part[r][kG + 1], gz = part[r][kG + 2];
+                tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]);
+                tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
+                tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
+                part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
+              }
+            }
+          } else {
+#pragma unroll kUnroll
+            for (int q = 0; q < SUB; ++q) {
+              const SrcR<Real, UNIF> s = lds[sb + q];
+              Real slo[3] = {Real(0), Real(0), Real(0)};
+              if constexpr (kStageLo) {
+                const float4 l = lds_lo[sb + q];
+                slo[0] = l.x;
+                slo[1] = l.y;
+                slo[2] = l.z;
+              }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              gsum[r][c] += static_cast<double>(part[r][kG + c]);
-              part[r][kG + c] = Real(0);
+              for (int r = 0; r < R; ++r)
+                pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, slo,
+                                                    eps_r2, kc);
             }
           }
+        }
+        if constexpr (kFold32) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                gsum[r][c] += static_cast<double>(part[r][kG + c]);
+                part[r][kG + c] = Real(0);
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < kNAcc; ++q) {
+              if (!S::used(q)) continue;
+              if constexpr (kComp) {  // Kahan: ksum + kcmp carries the tile sum
+                const Real y = part[r][q] - kcmp[r][q];
+                const Real z = ksum[r][q] + y;
+                kcmp[r][q] = (z - ksum[r][q]) - y;
+                ksum[r][q] = z;
+              } else {
+                tot[r][q] += static_cast<double>(part[r][q]);
+              }
+              part[r][q] = Real(0);
+            }
+          }
+        }
+      }
+      if constexpr (kComp) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
 #pragma unroll
           for (int q = 0; q < kNAcc; ++q) {
             if (!S::used(q)) continue;
-            if constexpr (kComp) {  // Kahan: ksum + kcmp carries the tile sum
-              const Real y = part[r][q] - kcmp[r][q];
-              const Real z = ksum[r][q] + y;
-              kcmp[r][q] = (z - ksum[r][q]) - y;
-              ksum[r][q] = z;
-            } else {
-              tot[r][q] += static_cast<double>(part[r][q]);
-            }
-            part[r][q] = Real(0);
+            tot[r][q] += static_cast<double>(ksum[r][q]) - static_cast<double>(kcmp[r][q]);
+            ksum[r][q] = Real(0);
+            kcmp[r][q] = Real(0);
           }
-        }
       }
     }
-    if constexpr (kComp) {
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int q = 0; q < kNAcc; ++q) {
-          if (!S::used(q)) continue;
-          tot[r][q] += static_cast<double>(ksum[r][q]) - static_cast<double>(kcmp[r][q]);
-          ksum[r][q] = Real(0);
-          kcmp[r][q] = Real(0);
-        }
-    }
-  }
+  };
+  if (g64)  // constant unless kBothG
+    sweep(std::true_type{});
+  else
+    sweep(std::false_type{});
   if constexpr (!kFold32) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -625,9 +776,24 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   }
 }
 
+// Minimum waves per SIMD the register allocation must allow.  The
+// velocity/stretching FP32 pass with 512 lanes x one target holds three
+// loops (FP64 G, FP32 G near, moment form far); left alone the allocator
+// settles just above 128 VGPRs, which would halve the workgroups per CU
+// (two of 512 lanes fit the LDS; smaller workgroups are LDS-bound anyway).
+template <typename Real, unsigned OUT, int BLOCK, int R, int TILE>
+constexpr int min_waves() {
+  using S = Slots<OUT>;
+  return (std::is_same<Real, float>::value && BLOCK == 512 && R == 1 && TILE <= 512 && S::str &&
+          !S::pse && !S::vort && !S::ens)
+             ? 4
+             : 1;
+}
+
 // grid (target blocks, splits)
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-__global__ __launch_bounds__(BLOCK) void fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE>())) void fused_kernel(
+    FusedArgs a) {
   const int k = static_cast<int>(blockIdx.y);
   const int lo = k * a.chunk;
   const int hi = a.chunk < a.n - lo ? lo + a.chunk : a.n;

@@ -275,3 +275,32 @@ def test_thin_ring_stretching_with_low_parts(gpu):
     st = P.strength_gradients(d[0], d[1], d[2])
     assert float((res["vel"] - u).norm() / u.norm()) <= 1e-6
     assert float((res["stretch"] - st).norm() / st.norm()) <= 1.5e-6
+
+
+def test_mixed_wave_modes_share_the_staged_tile(gpu):
+    """A ring of 256 stations inside a random cloud with the same radius:
+    sorted blocks where the two meet hold waves of both G modes (FP64 per
+    pair and FP32 partials with the far-field moment form), which share each
+    staged source tile.  Automatic mode must differ from both forced modes
+    (so the blocks do mix) and stay within the bar."""
+    xr, ar, sr, _ = FM.torus_ring(256, h=2 * np.pi / 256)
+    rng = np.random.default_rng(29)
+    nc = 3 * len(xr)
+    xc = np.stack([rng.uniform(-0.3, 0.3, nc), rng.uniform(-1.3, 1.3, nc),
+                   rng.uniform(-1.3, 1.3, nc)], -1)
+    ac = rng.normal(0.0, np.abs(ar).max() / 3, (nc, 3))
+    x = np.concatenate([xr, xc])
+    a = np.concatenate([ar, ac])
+    s = np.full(len(x), sr[0])
+    print(f"[mixed] n={len(x)}", flush=True)
+    d = to_dev(x, a, s, np.zeros(len(x)))
+    runs = {m: P.forward(*d, outputs=("vel", "stretch"), precision="f32c", sigma_mode="uniform",
+                         g64=m) for m in ("auto", "always", "never")}
+    u = P.selfinduced_velocity(d[0], d[1], d[2])
+    st = P.strength_gradients(d[0], d[1], d[2])
+    assert not torch.equal(runs["auto"]["stretch"], runs["always"]["stretch"])
+    assert not torch.equal(runs["auto"]["stretch"], runs["never"]["stretch"])
+    for m, res in runs.items():
+        assert torch.isfinite(res["vel"]).all() and torch.isfinite(res["stretch"]).all(), m
+    assert float((runs["auto"]["vel"] - u).norm() / u.norm()) <= 1e-6
+    assert float((runs["auto"]["stretch"] - st).norm() / st.norm()) <= 1e-6
