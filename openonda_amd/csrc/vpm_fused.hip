@@ -192,11 +192,19 @@ struct PairConst {
 
 // slo: the source's low parts (lx, ly, lz) when the PSE difference is taken
 // in two parts (FP32 pass), else null.
-template <typename Real, unsigned OUT, bool UNIF, bool G64>
+// SELF: the pair may be the target's own (self == true then); its term
+// g(0) alpha_i = 2.5 alpha_i leaves the FP32 partials of G.  alpha_i x G
+// annihilates it in exact arithmetic, but in FP32 partials it sets the size
+// of every rounding of its sub-tile, and in sparse sets (radius small
+// against the spacing) it dwarfs the other terms of G: a 37-particle cloud
+// lost the 1e-6 bar (2.5e-6) with it.  Velocity, stretching's r-term and
+// PSE vanish for the self pair; the enstrophy keeps it (the reference sums
+// i = j there).
+template <typename Real, unsigned OUT, bool UNIF, bool G64, bool SELF = false>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
                                             const SrcR<Real, UNIF>& s, const double* aj64,
                                             const Real* slo, Real eps_r2,
-                                            const PairConst<Real>& k) {
+                                            const PairConst<Real>& k, bool self = false) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
@@ -257,9 +265,10 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
         g64[1] = fma(gd, aj64[1], g64[1]);
         g64[2] = fma(gd, aj64[2], g64[2]);
       } else {
-        acc[kG + 0] += g * s.ax;
-        acc[kG + 1] += g * s.ay;
-        acc[kG + 2] += g * s.az;
+        const Real gG = (SELF && self) ? Real(0) : g;
+        acc[kG + 0] += gG * s.ax;
+        acc[kG + 1] += gG * s.ay;
+        acc[kG + 2] += gG * s.az;
       }
       Real e = t.ax * cx + t.ay * cy + t.az * cz;
       if constexpr (kLoCross) e += t.lx * cx + t.ly * cy + t.lz * cz;
@@ -403,13 +412,15 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // low parts alpha - fp32(alpha) of the strengths (PSE differences; the
   // stretching's triple product in coherent waves)
   constexpr bool kLo = (S::pse || S::str) && std::is_same<Real, float>::value;
-  // Velocity/stretching output sets of an FP32 pass carry two loops: FP64
-  // per-pair G for coherent waves, FP32 partials of G otherwise (with PSE,
-  // vorticity or enstrophy the second loop costs occupancy, and G is summed
-  // in FP64).  With velocity and a uniform radius the FP32-G waves also use
-  // the far-field moment form (pair_moment) on distant tiles.
-  constexpr bool kBothG = S::str && !S::pse && !S::vort && !S::ens;
-  constexpr bool kMoment = kBothG && S::vel && UNIF && std::is_same<Real, float>::value;
+  // Output sets with stretching carry two loops in an FP32 pass: FP64
+  // per-pair G for coherent waves, FP32 partials of G otherwise.  Sets with
+  // vorticity or enstrophy but no PSE are the exception: they unroll deeper
+  // and a second loop would cost occupancy, so they sum G in FP64 only.
+  // Velocity + stretching alone with a uniform radius also uses the far-field
+  // moment form (pair_moment) on distant tiles in FP32-G waves.
+  constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
+  constexpr bool kMoment = S::str && S::vel && !S::pse && !S::vort && !S::ens && UNIF &&
+                           std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
@@ -540,6 +551,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     constexpr bool kFar = kMoment && !G64;
     constexpr bool kStageLo = kLo;
     constexpr bool kStageA64 = S::str;
+    // FP32 partials of G: the sub-tiles holding this wave's own targets take
+    // the self pair out of G (pair_update, SELF).
+    constexpr bool kSelfMask = !G64 && S::str && std::is_same<Real, float>::value;
+    // first target of this wave, in a scalar register: the test below is
+    // then scalar arithmetic (per sub-tile, far tiles skip it)
+    const int wave0 = __builtin_amdgcn_readfirstlane(base + (tid & ~63));
+    auto self_sub = [&](int j0) {  // [j0, j0 + SUB) meets a wave's targets
+      bool meet = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int k0 = wave0 + r * BLOCK;
+        meet = meet || (j0 < k0 + 64 && k0 < j0 + SUB);
+      }
+      return meet;
+    };
     // Sources per unrolled step: deep unrolling lets the compiler prefetch LDS
     // records for 8 sources, which the heaviest output sets pay for in occupancy.
     constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
@@ -632,48 +658,87 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
         far = __builtin_amdgcn_readfirstlane(static_cast<int>(gap2 > kFarGap * kFarGap)) != 0;
       }
 
-      for (int sb = 0; sb < TILE; sb += SUB) {
-        if constexpr (G64) {
+      // Folds the sub-tile's FP32 partials: G into its FP64 totals, the others
+      // into the Kahan pairs (or FP64 totals).  FAR: a far tile adds nothing
+      // to u's partials (its velocity went to FP64 directly) and u's Kahan
+      // pair is zero throughout such a tile, so its update is skipped.
+      auto fold_sub = [&](auto far_tile) {
+        constexpr bool FAR = decltype(far_tile)::value;
+        if constexpr (kFold32) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                gsum[r][c] += static_cast<double>(part[r][kG + c]);
+                part[r][kG + c] = Real(0);
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < kNAcc; ++q) {
+              if (!S::used(q) || (FAR && q >= kU && q < kU + 3)) continue;
+              if constexpr (kComp) {  // Kahan: ksum + kcmp carries the tile sum
+                const Real y = part[r][q] - kcmp[r][q];
+                const Real z = ksum[r][q] + y;
+                kcmp[r][q] = (z - ksum[r][q]) - y;
+                ksum[r][q] = z;
+              } else {
+                tot[r][q] += static_cast<double>(part[r][q]);
+              }
+              part[r][q] = Real(0);
+            }
+          }
+        }
+      };
+
+      if (far) {  // constant false unless kFar: a far tile sweeps its sub-tiles alone
+#pragma unroll 1
+        for (int sb = 0; sb < TILE; sb += SUB) {
 #pragma unroll kUnroll
           for (int q = 0; q < SUB; ++q) {
             const SrcR<Real, UNIF> s = lds[sb + q];
-            const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1], lds_a64[sb + q][2]};
-            Real slo[3] = {Real(0), Real(0), Real(0)};
-            if constexpr (kStageLo) {
-              const float4 l = lds_lo[sb + q];
-              slo[0] = l.x;
-              slo[1] = l.y;
-              slo[2] = l.z;
-            }
+            const float4 m = make_float4(s.p0, s.p1, lds_mz[sb + q], 0.0f);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-              pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2,
-                                                 kc);
+            for (int r = 0; r < R; ++r) {
+              if constexpr (kFar) pair_moment<Real, UNIF>(part[r], t[r], pm[r], s, m, kc);
+            }
           }
-        } else {
-          if (far) {  // constant false unless kFar
+#pragma unroll
+          for (int r = 0; r < R; ++r) {  // u += x_i x G + H of the sub-tile, in FP64
+            if constexpr (kFar) {
+              const double X = lds_x64[r * BLOCK + tid][0], Y = lds_x64[r * BLOCK + tid][1],
+                           Z = lds_x64[r * BLOCK + tid][2];
+              const double gx = part[r][kG + 0], gy = part[r][kG + 1], gz = part[r][kG + 2];
+              tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]);
+              tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
+              tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
+              part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
+            }
+          }
+          fold_sub(std::integral_constant<bool, kFar>{});
+        }
+      } else {
+#pragma unroll 1
+        for (int sb = 0; sb < TILE; sb += SUB) {
+          if constexpr (G64) {
 #pragma unroll kUnroll
             for (int q = 0; q < SUB; ++q) {
               const SrcR<Real, UNIF> s = lds[sb + q];
-              const float4 m = make_float4(s.p0, s.p1, lds_mz[sb + q], 0.0f);
-#pragma unroll
-              for (int r = 0; r < R; ++r) {
-                if constexpr (kFar) pair_moment<Real, UNIF>(part[r], t[r], pm[r], s, m, kc);
+              const double aj64[3] = {lds_a64[sb + q][0], lds_a64[sb + q][1],
+                                      lds_a64[sb + q][2]};
+              Real slo[3] = {Real(0), Real(0), Real(0)};
+              if constexpr (kStageLo) {
+                const float4 l = lds_lo[sb + q];
+                slo[0] = l.x;
+                slo[1] = l.y;
+                slo[2] = l.z;
               }
-            }
 #pragma unroll
-            for (int r = 0; r < R; ++r) {  // u += x_i x G + H of the sub-tile, in FP64
-              if constexpr (kFar) {
-                const double X = lds_x64[r * BLOCK + tid][0], Y = lds_x64[r * BLOCK + tid][1],
-                             Z = lds_x64[r * BLOCK + tid][2];
-                const double gx = part[r][kG + 0], gy = part[r][kG + 1], gz = part[r][kG + 2];
-                tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]);
-                tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
-                tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
-                part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
-              }
+              for (int r = 0; r < R; ++r)
+                pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2,
+                                                   kc);
             }
-          } else {
+          } else if (!(kSelfMask && self_sub(t0 + sb))) {
 #pragma unroll kUnroll
             for (int q = 0; q < SUB; ++q) {
               const SrcR<Real, UNIF> s = lds[sb + q];
@@ -689,32 +754,26 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
                 pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, slo,
                                                     eps_r2, kc);
             }
-          }
-        }
-        if constexpr (kFold32) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                gsum[r][c] += static_cast<double>(part[r][kG + c]);
-                part[r][kG + c] = Real(0);
+          } else {  // the sub-tile holding this wave's targets: once per tile
+#pragma unroll 1
+            for (int q = 0; q < SUB; ++q) {
+              const SrcR<Real, UNIF> s = lds[sb + q];
+              Real slo[3] = {Real(0), Real(0), Real(0)};
+              if constexpr (kStageLo) {
+                const float4 l = lds_lo[sb + q];
+                slo[0] = l.x;
+                slo[1] = l.y;
+                slo[2] = l.z;
               }
-            }
+              const int j = t0 + sb + q;
 #pragma unroll
-            for (int q = 0; q < kNAcc; ++q) {
-              if (!S::used(q)) continue;
-              if constexpr (kComp) {  // Kahan: ksum + kcmp carries the tile sum
-                const Real y = part[r][q] - kcmp[r][q];
-                const Real z = ksum[r][q] + y;
-                kcmp[r][q] = (z - ksum[r][q]) - y;
-                ksum[r][q] = z;
-              } else {
-                tot[r][q] += static_cast<double>(part[r][q]);
-              }
-              part[r][q] = Real(0);
+              for (int r = 0; r < R; ++r)
+                pair_update<Real, OUT, UNIF, false, kSelfMask>(
+                    part[r], gsum[r], t[r], s, nullptr, slo, eps_r2, kc,
+                    j == base + r * BLOCK + tid);
             }
           }
+          fold_sub(std::false_type{});
         }
       }
       if constexpr (kComp) {

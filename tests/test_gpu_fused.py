@@ -204,14 +204,20 @@ def test_source_split_matches_single_pass(gpu, n, precision, mode, outputs):
     check({k: auto[k] for k in outputs}, ref, F64_TOL if precision == "f64" else F32C_TOL, "auto")
 
 
-def test_g64_modes_on_ring_and_cloud(gpu):
+@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse"),
+                                     ("vel", "stretch", "pse", "vort", "enstrophy")])
+def test_g64_modes_on_ring_and_cloud(gpu, outputs):
     """The stretching's alpha_j sum: FP64 per pair in coherent waves (a ring),
-    FP32 partials in incoherent ones (a random cloud); both within the bar."""
-    outputs = ("vel", "stretch")
+    FP32 partials in incoherent ones (a random cloud); both within the bar,
+    for velocity + stretching alone (with the far-field moment form), with
+    PSE, and with PSE and the LES outputs."""
     xr, ar, sr, nur = ring_set()
-    xc, ac, sc, nuc = FM.random_cloud(8192, seed=17)
-    ref_r = cached("ring", lambda: oracle_all(xr, ar, sr, nur))
-    ref_c = {"vel": O.selfinduced_velocity(xc, ac, sc), "stretch": O.strength_gradients(xc, ac, sc)}
+    xc, ac, sc, _ = FM.random_cloud(8192, seed=17)
+    nuc = np.full(len(xc), 1e-3)
+    ref_r = {k: v for k, v in cached("ring", lambda: oracle_all(xr, ar, sr, nur)).items()
+             if k in outputs}
+    ref_c = {k: v for k, v in cached("cloud8192", lambda: oracle_all(xc, ac, sc, nuc)).items()
+             if k in outputs}
     runs = {}
     for name, (x, a, s, nu) in (("ring", (xr, ar, sr, nur)), ("cloud", (xc, ac, sc, nuc))):
         d = to_dev(x, a, s, nu)
@@ -304,3 +310,31 @@ def test_mixed_wave_modes_share_the_staged_tile(gpu):
         assert torch.isfinite(res["vel"]).all() and torch.isfinite(res["stretch"]).all(), m
     assert float((runs["auto"]["vel"] - u).norm() / u.norm()) <= 1e-6
     assert float((runs["auto"]["stretch"] - st).norm() / st.norm()) <= 1e-6
+
+
+@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse"),
+                                     ("vel", "stretch", "pse", "vort", "enstrophy")])
+def test_sparse_cloud_keeps_the_self_pair_out_of_g(gpu, outputs):
+    """Radius 0.05 against a spacing of ~0.3: the alpha_j sum G of each target
+    is dominated by its own pair (2.5 alpha_i), which alpha_i x G cancels.
+    The FP32-partial waves must keep that term out of their partials (the
+    self-pair mask); a few hundred particles span several waves, sources
+    tiles and both radius-mode kernels.  Velocity and stretching are checked
+    (the outputs G enters); the other outputs ride along so that each output
+    set's kernel runs.  (PSE on such a set is a sum of one or two t^9 terms
+    and carries their FP32 roundings unaveraged: ~3e-6 with per-particle
+    radii.)"""
+    rng = np.random.default_rng(31)
+    for n in (37, 300):
+        x = rng.uniform(-1.0, 1.0, (n, 3))
+        a = rng.normal(0.0, 0.03, (n, 3))
+        s = np.full(n, 0.05)
+        nu = np.full(n, 1e-3)
+        ref = oracle_all(x, a, s, nu)
+        ref = {k: v for k, v in ref.items() if k in outputs}
+        for mode in ("uniform", "general"):
+            print(f"[sparse] n={n} {mode} {outputs}", flush=True)
+            res = P.forward(*to_dev(x, a, s, nu), outputs=outputs, precision="f32c",
+                            sigma_mode=mode, g64="never")
+            check({k: res[k] for k in ("vel", "stretch")}, ref, F32C_TOL,
+                  f"sparse n={n} {mode}")
