@@ -192,7 +192,8 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
  * autotuner).  Writes block threads, targets per lane, source tile, sub-tile
  * length and compensation mode (0 Kahan+FP64, 1 direct FP64 fold). */
 int vpm_tile_config_count(void);
-/* index used for tile_config = -1 with these outputs: one set / a batch (FP32) */
+/* index used for tile_config = -1 with these outputs: one set with a uniform
+ * radius / a batch (FP32) */
 int vpm_tile_config_default(uint32_t outputs);
 int vpm_batch_tile_config_default(uint32_t outputs);
 int vpm_tile_config_info(int index, int* block, int* targets_per_lane, int* tile,

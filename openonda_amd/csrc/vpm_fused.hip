@@ -897,35 +897,46 @@ __global__ __launch_bounds__(256) void split_reduce_kernel(FusedArgs a, int nspl
 struct TileCfg {
   int block, r, tile, sub, comp;
 };
-// Index 0 is the FP32 default without PSE (the autotuner's winner for
-// velocity + stretching on a random cloud of 2^20: 512 lanes x one target,
-// 64-source FP32 partials into a Kahan pair per tile); index 1 the default
-// with PSE (the winner for velocity + stretching + PSE on a ring of 131072:
-// 256 lanes, 512-source tiles).  Both exist for every output set; configs
-// 2.. for the hot output sets (VEL|STRETCH, VEL|STRETCH|PSE,
-// VEL|STRETCH|VORT|ENSTROPHY) with uniform sigma.
+// Configs 0 and 1 exist for every output set; configs 2.. for the hot
+// output sets (VEL|STRETCH, VEL|STRETCH|PSE, VEL|STRETCH|VORT|ENSTROPHY) with
+// uniform sigma.
 constexpr TileCfg kCfgs[] = {
     {512, 1, 512, 64, 0},  {256, 1, 512, 64, 0},  {256, 2, 256, 32, 1}, {256, 1, 256, 64, 0},
     {256, 1, 256, 64, 1},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
     {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},
 };
 
-constexpr int default_config(unsigned kernel_mask) {
-  return (kernel_mask & VPM_OUT_PSE) ? 1 : 0;
+constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
+
+// Shapes for tile_config = -1, the autotuner's winners on one MI355X:
+//  * velocity + stretching, uniform sigma: config 5 (512 lanes x one target,
+//    128-source FP32 partials: half the folds of config 0; random cloud of
+//    2^20: 685 ms against 693 ms, rel-L2 5.0e-7 against 4.5e-7);
+//  * velocity + stretching + PSE: config 0 (cloud of 131072: 17.1 ms against
+//    17.8 ms for config 1; ring of 131072: 21.3 against 21.6 ms);
+//  * other sets: config 1 (256 lanes) with PSE -- their kernels need more than
+//    128 VGPRs, where a 512-lane workgroup would run alone on its CU -- and
+//    config 0 without.
+constexpr int default_config(unsigned kernel_mask, bool unif) {
+  return (unif && kernel_mask == 3u)   ? 5
+         : kernel_mask == 7u           ? 0
+         : (kernel_mask & VPM_OUT_PSE) ? 1
+                                       : 0;
 }
 
-// The batched launch uses the same shapes (96 sets, 3.3e6 particles, RK4 with
-// PSE and LES: 1068 ms per step with config 1, 1070-1092 ms with the other
-// one-target shapes, 1081-1205 ms with two targets per lane).
-constexpr int default_batch_config(unsigned batch_mask) { return default_config(batch_mask); }
-
-constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
+// The batched launch: 256 lanes with PSE (96 sets, 3.3e6 particles, RK4 with
+// PSE and LES: 1068 ms per step with config 1 before the cloud sets ran the
+// FP32-partial loop, 1070-1092 ms with the other one-target shapes,
+// 1081-1205 ms with two targets per lane), 512 without.
+constexpr int default_batch_config(unsigned batch_mask) {
+  return (batch_mask & VPM_OUT_PSE) ? 1 : 0;
+}
 
 // The configuration the single-set launch really uses for a request.
 constexpr int effective_config(unsigned kernel_mask, bool unif, int requested) {
-  return requested < 0 ? default_config(kernel_mask)
+  return requested < 0 ? default_config(kernel_mask, unif)
          : (requested <= 1 || (unif && hot_mask(kernel_mask))) ? requested
-                                                                : default_config(kernel_mask);
+                                                                : default_config(kernel_mask, unif);
 }
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 constexpr TileCfg kCfgF64 = {256, 1, 256, 256, 1};
@@ -1111,7 +1122,8 @@ struct BatchArgs {
 };
 
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-__global__ __launch_bounds__(BLOCK) void fused_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE>())) void fused_batch_kernel(
+    BatchArgs b) {
   __shared__ int item_s;
   for (;;) {
     __syncthreads();
@@ -1384,7 +1396,7 @@ extern "C" {
 int vpm_tile_config_count(void) { return kNumCfgs; }
 
 int vpm_tile_config_default(uint32_t outputs) {
-  return default_config(kernel_mask(outputs == 0 ? VPM_OUT_VEL : outputs));
+  return default_config(kernel_mask(outputs == 0 ? VPM_OUT_VEL : outputs), true);
 }
 
 int vpm_batch_tile_config_default(uint32_t outputs) {

@@ -280,7 +280,7 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
 
 
 def tile_config_default(outputs=("vel", "stretch")):
-    """Index of the tile configuration used for ``tile_config=-1``."""
+    """Index of the tile configuration used for ``tile_config=-1`` (uniform radius)."""
     if isinstance(outputs, str):
         outputs = (outputs,)
     mask = 0
