@@ -111,9 +111,9 @@ struct Tgt {
   Real x, y, z, ax, ay, az, hs, lx, ly, lz;
 };
 
-// Accumulator slots.  kH, the moment sum of the far-field form, shares the
-// PSE slots: that form runs only for output sets without PSE.
-enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kH = 9, kW = 12, kE = 15, kNAcc = 18 };
+// Accumulator slots; kH is the moment sum of the far-field form (folded into
+// kU's FP64 totals per sub-tile, never stored).
+enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kW = 12, kE = 15, kH = 18, kNAcc = 21 };
 
 template <unsigned OUT>
 struct Slots {
@@ -320,9 +320,9 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   }
 }
 
-// Far-field moment form of velocity + stretching (FP32, uniform radius,
-// waves whose strengths are unrelated).  With d = x_i - x_j and coordinates
-// relative to the workgroup centre,
+// Far-field moment form of velocity + stretching (+ PSE), FP32, uniform
+// radius.  With d = x_i - x_j and coordinates relative to the workgroup
+// centre,
 //   sum_j g d x a_j  = x_i x G + sum_j g (a_j x x_j),   G = sum_j g a_j,
 //   a_i . (d x a_j)  = a_j . (a_i x x_i) + a_i . (a_j x x_j),
 // so with G already summed for the stretching and the source moments
@@ -333,13 +333,19 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
 // more than kFarGap radii from the workgroup's targets, where it stays
 // small (random cloud of 2^20 in FP32 emulation: rel-L2 unchanged at
 // 3.1e-7 / 6.2e-7 with 97% of the pairs in the far form; using it for
-// every pair gives 1.2e-6 / 8.7e-7).  p = a_i x x_i per target.
+// every pair gives 1.2e-6 / 8.7e-7).  Coherent waves (vortex tubes) use it
+// on far tiles too, with FP32 partials of G there: far terms of G are small
+// beside the near ones that alpha_i x G cancels, and the triple product's
+// two moment terms are each small along a tube (thin ring of 131072 in
+// emulation: 1.27e-7 / 1.02e-6 either way, 88% of pairs far).
+// p = a_i x x_i per target.
 constexpr double kFarGap = 4.0;
 
-template <typename Real, bool UNIF>
+template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const Real* p,
                                             const SrcR<Real, UNIF>& s, const float4& m,
-                                            const PairConst<Real>& k) {
+                                            const Real* slo, const PairConst<Real>& k) {
+  using S = Slots<OUT>;
   using M = Arith<Real>;
   const Real dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
   const Real A = M::fma(dz, dz, M::fma(dy, dy, M::fma(dx, dx, Real(1))));
@@ -364,6 +370,15 @@ __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const
   acc[kSd + 0] = M::fma(w, dx, acc[kSd + 0]);
   acc[kSd + 1] = M::fma(w, dy, acc[kSd + 1]);
   acc[kSd + 2] = M::fma(w, dz, acc[kSd + 2]);
+  if constexpr (S::pse) {  // as in pair_update: alpha_j - alpha_i from high and low parts
+    const Real t9 = t5 * t2 * t2;
+    const Real dax = (s.ax - t.ax) + (slo[0] - t.lx);
+    const Real day = (s.ay - t.ay) + (slo[1] - t.ly);
+    const Real daz = (s.az - t.az) + (slo[2] - t.lz);
+    acc[kP + 0] = M::fma(t9, dax, acc[kP + 0]);
+    acc[kP + 1] = M::fma(t9, day, acc[kP + 1]);
+    acc[kP + 2] = M::fma(t9, daz, acc[kP + 2]);
+  }
 }
 
 // Coherence of 64 consecutive sorted targets (one lane row of a wave),
@@ -419,7 +434,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // Velocity + stretching alone with a uniform radius also uses the far-field
   // moment form (pair_moment) on distant tiles in FP32-G waves.
   constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
-  constexpr bool kMoment = S::str && S::vel && !S::pse && !S::vort && !S::ens && UNIF &&
+  constexpr bool kMoment = S::str && S::vel && !S::vort && !S::ens && UNIF &&
                            std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
@@ -548,7 +563,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // wave stages (and bounds) its share of every LDS array either mode reads.
   auto sweep = [&](auto mode) {
     constexpr bool G64 = decltype(mode)::value;
-    constexpr bool kFar = kMoment && !G64;
+    constexpr bool kFar = kMoment;
     constexpr bool kStageLo = kLo;
     constexpr bool kStageA64 = S::str;
     // FP32 partials of G: the sub-tiles holding this wave's own targets take
@@ -698,9 +713,16 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           for (int q = 0; q < SUB; ++q) {
             const SrcR<Real, UNIF> s = lds[sb + q];
             const float4 m = make_float4(s.p0, s.p1, lds_mz[sb + q], 0.0f);
+            Real slo[3] = {Real(0), Real(0), Real(0)};
+            if constexpr (S::pse && kStageLo) {
+              const float4 l = lds_lo[sb + q];
+              slo[0] = l.x;
+              slo[1] = l.y;
+              slo[2] = l.z;
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-              if constexpr (kFar) pair_moment<Real, UNIF>(part[r], t[r], pm[r], s, m, kc);
+              if constexpr (kFar) pair_moment<Real, OUT, UNIF>(part[r], t[r], pm[r], s, m, slo, kc);
             }
           }
 #pragma unroll
@@ -844,7 +866,7 @@ template <typename Real, unsigned OUT, int BLOCK, int R, int TILE>
 constexpr int min_waves() {
   using S = Slots<OUT>;
   return (std::is_same<Real, float>::value && BLOCK == 512 && R == 1 && TILE <= 512 && S::str &&
-          !S::pse && !S::vort && !S::ens)
+          !S::vort && !S::ens)
              ? 4
              : 1;
 }
@@ -912,24 +934,26 @@ constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
 //  * velocity + stretching, uniform sigma: config 5 (512 lanes x one target,
 //    128-source FP32 partials: half the folds of config 0; random cloud of
 //    2^20: 685 ms against 693 ms, rel-L2 5.0e-7 against 4.5e-7);
-//  * velocity + stretching + PSE: config 0 (cloud of 131072: 17.1 ms against
-//    17.8 ms for config 1; ring of 131072: 21.3 against 21.6 ms);
+//  * velocity + stretching + PSE, uniform sigma: config 4 (256 lanes,
+//    256-source tiles, partials folded straight into FP64; with the moment
+//    form on far tiles: ring of 131072 18.6 ms against 19.4 for config 1
+//    and 20.2 for config 0; cloud of 131072 16.9 against 17.0 and 16.3);
 //  * other sets: config 1 (256 lanes) with PSE -- their kernels need more than
 //    128 VGPRs, where a 512-lane workgroup would run alone on its CU -- and
 //    config 0 without.
 constexpr int default_config(unsigned kernel_mask, bool unif) {
   return (unif && kernel_mask == 3u)   ? 5
-         : kernel_mask == 7u           ? 0
+         : (unif && kernel_mask == 7u) ? 4
          : (kernel_mask & VPM_OUT_PSE) ? 1
                                        : 0;
 }
 
-// The batched launch: 256 lanes with PSE (96 sets, 3.3e6 particles, RK4 with
-// PSE and LES: 1068 ms per step with config 1 before the cloud sets ran the
-// FP32-partial loop, 1070-1092 ms with the other one-target shapes,
-// 1081-1205 ms with two targets per lane), 512 without.
+// The batched launch: config 4 with PSE (96 sets, 3.3e6 particles, RK4 with
+// PSE and LES, far tiles in the moment form: 882 ms per step against 893
+// for config 1, 896 for config 3 and 956 for config 0 on one box), config 0
+// without.
 constexpr int default_batch_config(unsigned batch_mask) {
-  return (batch_mask & VPM_OUT_PSE) ? 1 : 0;
+  return (batch_mask & VPM_OUT_PSE) ? 4 : 0;
 }
 
 // The configuration the single-set launch really uses for a request.
