@@ -57,8 +57,10 @@ class BatchedSystem:
         self.precision = precision
         self.pse = pse
         self.les = les
-        # one tile shape for every stage: the work items count its target blocks
-        stage = L.OUT_VEL | L.OUT_STRETCH | (L.OUT_PSE if pse else 0)
+        # one tile shape for every stage (the work items count its target
+        # blocks), chosen for the heaviest stage's outputs
+        stage = (L.OUT_VEL | L.OUT_STRETCH | (L.OUT_PSE if pse else 0)
+                 | (L.OUT_VORT | L.OUT_ENSTROPHY if les else 0))
         self.tile_config = (int(tile_config) if tile_config >= 0
                             else int(L.load().vpm_batch_tile_config_default(stage)))
         sizes = [len(s[0]) for s in sets]

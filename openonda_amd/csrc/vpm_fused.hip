@@ -370,6 +370,18 @@ __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const
   acc[kSd + 0] = M::fma(w, dx, acc[kSd + 0]);
   acc[kSd + 1] = M::fma(w, dy, acc[kSd + 1]);
   acc[kSd + 2] = M::fma(w, dz, acc[kSd + 2]);
+  if constexpr (S::vort) {  // uniform radius: t^7; r^2 > 1e-8 always holds on a far tile
+    const Real wv = t5 * t2;
+    acc[kW + 0] = M::fma(wv, s.ax, acc[kW + 0]);
+    acc[kW + 1] = M::fma(wv, s.ay, acc[kW + 1]);
+    acc[kW + 2] = M::fma(wv, s.az, acc[kW + 2]);
+  }
+  if constexpr (S::ens) {
+    const Real en = M::fma(k.c22, t3, t1);
+    acc[kE + 0] = M::fma(en, s.ax, acc[kE + 0]);
+    acc[kE + 1] = M::fma(en, s.ay, acc[kE + 1]);
+    acc[kE + 2] = M::fma(en, s.az, acc[kE + 2]);
+  }
   if constexpr (S::pse) {  // as in pair_update: alpha_j - alpha_i from high and low parts
     const Real t9 = t5 * t2 * t2;
     const Real dax = (s.ax - t.ax) + (slo[0] - t.lx);
@@ -434,8 +446,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // Velocity + stretching alone with a uniform radius also uses the far-field
   // moment form (pair_moment) on distant tiles in FP32-G waves.
   constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
-  constexpr bool kMoment = S::str && S::vel && !S::vort && !S::ens && UNIF &&
-                           std::is_same<Real, float>::value;
+  constexpr bool kMoment = S::str && S::vel && UNIF && std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
@@ -865,10 +876,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 template <typename Real, unsigned OUT, int BLOCK, int R, int TILE>
 constexpr int min_waves() {
   using S = Slots<OUT>;
-  return (std::is_same<Real, float>::value && BLOCK == 512 && R == 1 && TILE <= 512 && S::str &&
-          !S::vort && !S::ens)
-             ? 4
-             : 1;
+  if (!std::is_same<Real, float>::value || R != 1 || TILE > 512 || !S::str) return 1;
+  if (!S::vort && !S::ens) return BLOCK == 512 ? 4 : 1;
+  return BLOCK == 256 ? 3 : 1;  // LES sets: three workgroups of 256 lanes per CU
 }
 
 // grid (target blocks, splits)
@@ -938,22 +948,23 @@ constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
 //    256-source tiles, partials folded straight into FP64; with the moment
 //    form on far tiles: ring of 131072 18.6 ms against 19.4 for config 1
 //    and 20.2 for config 0; cloud of 131072 16.9 against 17.0 and 16.3);
-//  * other sets: config 1 (256 lanes) with PSE -- their kernels need more than
-//    128 VGPRs, where a 512-lane workgroup would run alone on its CU -- and
-//    config 0 without.
+//  * other sets: config 1 (256 lanes) with PSE, vorticity or enstrophy --
+//    their kernels need more than 128 VGPRs, where a 512-lane workgroup would
+//    run alone on its CU -- and config 0 without.
+constexpr unsigned kHeavyOutputs = VPM_OUT_PSE | VPM_OUT_VORT | VPM_OUT_ENSTROPHY;
 constexpr int default_config(unsigned kernel_mask, bool unif) {
-  return (unif && kernel_mask == 3u)   ? 5
-         : (unif && kernel_mask == 7u) ? 4
-         : (kernel_mask & VPM_OUT_PSE) ? 1
-                                       : 0;
+  return (unif && kernel_mask == 3u)      ? 5
+         : (unif && kernel_mask == 7u)    ? 4
+         : (kernel_mask & kHeavyOutputs) ? 1
+                                          : 0;
 }
 
-// The batched launch: config 4 with PSE (96 sets, 3.3e6 particles, RK4 with
-// PSE and LES, far tiles in the moment form: 882 ms per step against 893
-// for config 1, 896 for config 3 and 956 for config 0 on one box), config 0
-// without.
+// The batched launch: config 4 with PSE or the LES outputs (96 sets, 3.3e6
+// particles, RK4 with PSE and LES, far tiles in the moment form: 882 ms per
+// step against 893 for config 1, 896 for config 3 and 956 for config 0 on
+// one box), config 0 for velocity + stretching alone.
 constexpr int default_batch_config(unsigned batch_mask) {
-  return (batch_mask & VPM_OUT_PSE) ? 4 : 0;
+  return (batch_mask & kHeavyOutputs) ? 4 : 0;
 }
 
 // The configuration the single-set launch really uses for a request.

@@ -78,14 +78,16 @@ def test_tile_config_defaults():
     """tile_config = -1 picks one shape per output set: 512 lanes with
     128-source partials for velocity + stretching, 256 lanes with 256-source
     tiles and direct FP64 folds for velocity + stretching + PSE, 256 lanes
-    for the other sets with PSE; the batch uses the PSE shape of velocity +
-    stretching + PSE with PSE and 512 lanes without."""
+    for the other sets with PSE or the LES outputs; the batch uses the shape
+    of velocity + stretching + PSE with PSE or LES, 512 lanes without."""
     lib = L.load()
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 5
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 4
     assert lib.vpm_tile_config_default(31) == 1
+    assert lib.vpm_tile_config_default(27) == 1
     assert lib.vpm_tile_config_default(L.OUT_VEL) == 0
     assert lib.vpm_batch_tile_config_default(31) == 4
+    assert lib.vpm_batch_tile_config_default(27) == 4
     assert lib.vpm_batch_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 0
     vals = [ctypes.c_int() for _ in range(5)]
     for idx, want_block in ((0, 512), (1, 256)):
