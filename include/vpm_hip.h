@@ -154,9 +154,11 @@ typedef struct vpm_forward_opts {
                          * are split among workgroups and the FP64 partial sums
                          * are added in split order (deterministic). */
   int32_t g64_mode;     /* VPM_G64_*: how the FP32 pass accumulates the stretching's
-                         * sum_j g alpha_j for the output sets VEL|STRETCH and
-                         * STRETCH (with PSE, VORT or ENSTROPHY it is always FP64;
-                         * ignored by the FP64 pass) */
+                         * sum_j g alpha_j on source tiles near the targets, for
+                         * output sets with STRETCH (sets with VORT or ENSTROPHY
+                         * and without PSE: always FP64; far tiles of a uniform
+                         * radius with VEL: always FP32 partials; ignored by the
+                         * FP64 pass) */
 } vpm_forward_opts_t;
 
 /* VPM_G64_AUTO: FP64 per pair in waves with a row of 64 consecutive sorted

@@ -211,9 +211,11 @@ def forward(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), precision="f
     ``max_splits``: 0 lets the library split the sources among workgroups
     when the targets alone cannot fill the device (small N); k >= 1 caps
     the number of splits (1: never split).
-    ``g64``: how the "f32c" pass sums the stretching's g alpha_j -- "auto"
-    (FP64 per pair in 64-lane waves of coherent target strengths, FP32
-    partials elsewhere), "always" or "never".
+    ``g64``: how the "f32c" pass sums the stretching's g alpha_j over source
+    tiles near the targets -- "auto" (FP64 per pair in 64-lane waves of
+    coherent target strengths, FP32 partials elsewhere), "always" or
+    "never"; distant tiles of a uniform-radius pass with velocity take FP32
+    partials in every mode.
     ``out`` may supply preallocated tensors by name.
     """
     if isinstance(outputs, str):
