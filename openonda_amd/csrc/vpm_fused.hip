@@ -203,7 +203,7 @@ struct PairConst {
 template <typename Real, unsigned OUT, bool UNIF, bool G64, bool SELF = false>
 __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Real>& t,
                                             const SrcR<Real, UNIF>& s, const double* aj64,
-                                            const Real* slo, Real eps_r2,
+                                            const double* ai64, const Real* slo, Real eps_r2,
                                             const PairConst<Real>& k, bool self = false) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
@@ -242,17 +242,15 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
     Real cx = dy * s.az - dz * s.ay;
     Real cy = dz * s.ax - dx * s.az;
     Real cz = dx * s.ay - dy * s.ax;
-    // Coherent waves (G64) of an FP32 pass with stretching: c = r x alpha_j
-    // and e = alpha_i . c also take the strengths' low parts.  Along a vortex
-    // tube alpha_i, r and alpha_j are nearly coplanar and e is a small
-    // triple product, so the rounding of the FP32 strengths (~eps |alpha|)
-    // would dominate it (thin rings: rel-L2 4e-6 -> 9e-7 in emulation).
-    constexpr bool kLoCross = G64 && S::str && std::is_same<Real, float>::value;
-    if constexpr (kLoCross) {
-      cx = M::fma(dy, slo[2], M::fma(-dz, slo[1], cx));
-      cy = M::fma(dz, slo[0], M::fma(-dx, slo[2], cy));
-      cz = M::fma(dx, slo[1], M::fma(-dy, slo[0], cz));
-    }
+    // Coherent waves (G64) of an FP32 pass with stretching evaluate the
+    // triple product e = alpha_i . (r x alpha_j) in FP64, from the FP32
+    // offsets and the FP64 strengths.  Along a vortex tube alpha_i, r and
+    // alpha_j are nearly coplanar and e is a small difference of products of
+    // size |alpha|^2 |r|; in FP32 its rounding dominated the stretching (thin
+    // ring of 126592 in emulation: 4e-6 with FP32 strengths, 8.9e-7 with
+    // their low parts in FP32, 3.4e-7 in FP64).  The velocity's c stays FP32
+    // (8.8e-8 there either way).
+    constexpr bool kE64 = G64 && S::str && std::is_same<Real, float>::value;
     if constexpr (S::vel) {
       acc[kU + 0] += g * cx;
       acc[kU + 1] += g * cy;
@@ -270,8 +268,16 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
         acc[kG + 1] += gG * s.ay;
         acc[kG + 2] += gG * s.az;
       }
-      Real e = t.ax * cx + t.ay * cy + t.az * cz;
-      if constexpr (kLoCross) e += t.lx * cx + t.ly * cy + t.lz * cz;
+      Real e;
+      if constexpr (kE64) {
+        const double rx = dx, ry = dy, rz = dz;
+        const double qx = ry * aj64[2] - rz * aj64[1];
+        const double qy = rz * aj64[0] - rx * aj64[2];
+        const double qz = rx * aj64[1] - ry * aj64[0];
+        e = static_cast<Real>(ai64[0] * qx + ai64[1] * qy + ai64[2] * qz);
+      } else {
+        e = t.ax * cx + t.ay * cy + t.az * cz;
+      }
       Real f2 = t5 * M::fma(k.c75, t2, k.c3);
       if constexpr (!UNIF) f2 *= inv2 * inv2 * inv;
       const Real w = f2 * e;
@@ -436,9 +442,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
   constexpr bool kFold32 = std::is_same<Real, float>::value;
-  // low parts alpha - fp32(alpha) of the strengths (PSE differences; the
-  // stretching's triple product in coherent waves)
-  constexpr bool kLo = (S::pse || S::str) && std::is_same<Real, float>::value;
+  // low parts alpha - fp32(alpha) of the strengths (PSE differences)
+  constexpr bool kLo = S::pse && std::is_same<Real, float>::value;
   // Output sets with stretching carry two loops in an FP32 pass: FP64
   // per-pair G for coherent waves, FP32 partials of G otherwise.  Sets with
   // vorticity or enstrophy but no PSE are the exception: they unroll deeper
@@ -521,6 +526,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       t[r].ly = static_cast<Real>(tg[r].ay - static_cast<double>(t[r].ay));
       t[r].lz = static_cast<Real>(tg[r].az - static_cast<double>(t[r].az));
     }
+  }
+  double a64t[R][3];  // the targets' strengths in FP64 (triple products of coherent waves)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    a64t[r][0] = tg[r].ax;
+    a64t[r][1] = tg[r].ay;
+    a64t[r][2] = tg[r].az;
   }
   Real pm[R][3];  // a_i x x_i of the moment form
 #pragma unroll
@@ -768,8 +780,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
               }
 #pragma unroll
               for (int r = 0; r < R; ++r)
-                pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, slo, eps_r2,
-                                                   kc);
+                pair_update<Real, OUT, UNIF, true>(part[r], gsum[r], t[r], s, aj64, a64t[r], slo,
+                                                   eps_r2, kc);
             }
           } else if (!(kSelfMask && self_sub(t0 + sb))) {
 #pragma unroll kUnroll
@@ -784,8 +796,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
               }
 #pragma unroll
               for (int r = 0; r < R; ++r)
-                pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, slo,
-                                                    eps_r2, kc);
+                pair_update<Real, OUT, UNIF, false>(part[r], gsum[r], t[r], s, nullptr, nullptr,
+                                                    slo, eps_r2, kc);
             }
           } else {  // the sub-tile holding this wave's targets: once per tile
 #pragma unroll 1
@@ -802,7 +814,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 #pragma unroll
               for (int r = 0; r < R; ++r)
                 pair_update<Real, OUT, UNIF, false, kSelfMask>(
-                    part[r], gsum[r], t[r], s, nullptr, slo, eps_r2, kc,
+                    part[r], gsum[r], t[r], s, nullptr, nullptr, slo, eps_r2, kc,
                     j == base + r * BLOCK + tid);
             }
           }
@@ -877,8 +889,10 @@ template <typename Real, unsigned OUT, int BLOCK, int R, int TILE>
 constexpr int min_waves() {
   using S = Slots<OUT>;
   if (!std::is_same<Real, float>::value || R != 1 || TILE > 512 || !S::str) return 1;
-  if (!S::vort && !S::ens) return BLOCK == 512 ? 4 : 1;
-  return BLOCK == 256 ? 3 : 1;  // LES sets: three workgroups of 256 lanes per CU
+  if (BLOCK == 512) return (!S::vort && !S::ens) ? 4 : 1;
+  // PSE and LES sets of 256 lanes: three workgroups per CU
+  if (BLOCK == 256) return (S::pse || S::vort || S::ens) ? 3 : 1;
+  return 1;
 }
 
 // grid (target blocks, splits)

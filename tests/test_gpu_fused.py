@@ -268,11 +268,11 @@ def test_library_autotune_entry_point(gpu):
     assert 0 <= best < len(ms) and ms[best] == min(ms)
 
 
-def test_thin_ring_stretching_with_low_parts(gpu):
+def test_thin_ring_stretching_with_fp64_triple_product(gpu):
     """A thin ring (1978 stations, core 3h ~ 0.01, sigma = 1.3h) has
     |S| ~ 2e-3 |alpha||u|: the stretching is a near-cancellation.  Coherent
-    waves feed the strengths' low parts into the triple product; without
-    them the FP32 pass lands at ~4e-6."""
+    waves take the triple product in FP64 on near tiles (4.8e-7 on the GPU;
+    FP32 strengths gave ~4e-6, their low parts in FP32 ~1e-6)."""
     x, a, s, nu = FM.torus_ring(1978, h=2 * np.pi / 1978)
     print(f"[thin-ring] n={len(x)}", flush=True)
     d = to_dev(x, a, s, nu)
@@ -280,7 +280,7 @@ def test_thin_ring_stretching_with_low_parts(gpu):
     u = P.selfinduced_velocity(d[0], d[1], d[2])
     st = P.strength_gradients(d[0], d[1], d[2])
     assert float((res["vel"] - u).norm() / u.norm()) <= 1e-6
-    assert float((res["stretch"] - st).norm() / st.norm()) <= 1.5e-6
+    assert float((res["stretch"] - st).norm() / st.norm()) <= 1e-6
 
 
 def test_mixed_wave_modes_share_the_staged_tile(gpu):
