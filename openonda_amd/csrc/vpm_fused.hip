@@ -19,6 +19,9 @@
 //    relative to the block size instead of to |x|.  With uniform sigma the
 //    offsets are pre-scaled by 1/sigma, so r^2 is rho^2 directly.
 //  * Each lane reads the same LDS record (broadcast) and updates R targets.
+//  * Per source tile and wave, one of three loops: the far-field moment form
+//    (tile far from the workgroup's targets), FP64 per-pair G with an FP64
+//    triple product (waves of coherent strengths), or FP32 partials of G.
 //  * Accumulation: FP32 partials over SUB sources; partials enter a Kahan
 //    (sum, compensation) pair per tile (COMP=0) and the compensated tile sum
 //    is folded into FP64 totals; COMP=1 folds partials straight into FP64.
@@ -122,10 +125,11 @@ struct Slots {
                         ens = OUT & VPM_OUT_ENSTROPHY;
   // Slots with FP64 totals.  The stretching's alpha_j sum G has FP64
   // totals of its own (gsum): waves whose targets' strengths are coherent
-  // (vortex rings, tubes, sheets) add to it in FP64 per pair, because there
-  // alpha_i x G is a near-exact cancellation and FP32 partials of G cost an
-  // order of magnitude in the stretching's relative error (coherent_wave
-  // below); other waves fold FP32 partials kG..kG+2 into it per sub-tile.
+  // (vortex rings, tubes, sheets) add to it in FP64 per pair on near tiles,
+  // because there alpha_i x G is a near-exact cancellation and FP32 partials
+  // of G cost an order of magnitude in the stretching's relative error
+  // (coherent_wave below); other waves, and every wave on far tiles, fold
+  // FP32 partials kG..kG+2 into it per sub-tile.
   static __host__ __device__ constexpr bool used(int k) {
     return (k < 3 && vel) || (k >= 6 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
            (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
@@ -444,12 +448,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   constexpr bool kFold32 = std::is_same<Real, float>::value;
   // low parts alpha - fp32(alpha) of the strengths (PSE differences)
   constexpr bool kLo = S::pse && std::is_same<Real, float>::value;
-  // Output sets with stretching carry two loops in an FP32 pass: FP64
-  // per-pair G for coherent waves, FP32 partials of G otherwise.  Sets with
-  // vorticity or enstrophy but no PSE are the exception: they unroll deeper
-  // and a second loop would cost occupancy, so they sum G in FP64 only.
-  // Velocity + stretching alone with a uniform radius also uses the far-field
-  // moment form (pair_moment) on distant tiles in FP32-G waves.
+  // Output sets with stretching carry two near-tile loops in an FP32 pass:
+  // FP64 per-pair G (and triple product) for coherent waves, FP32 partials of
+  // G otherwise.  Sets with vorticity or enstrophy but no PSE are the
+  // exception: they unroll deeper and a second loop would cost occupancy, so
+  // they sum G in FP64 only.  With velocity and a uniform radius every wave
+  // runs the far-field moment form (pair_moment) on distant tiles.
   constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
   constexpr bool kMoment = S::str && S::vel && UNIF && std::is_same<Real, float>::value;
   __shared__ SrcR<Real, UNIF> lds[TILE];
