@@ -65,8 +65,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the pair sum shards by targets; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the pair sum "
+              "would shard by targets, each GPU holding all sources)", file=sys.stderr)
         return 2
 
     import numpy as np
