@@ -1,7 +1,7 @@
 """Build recipe of the CPU baseline library oracle/libvpm_cpu.so (test infrastructure).
 
 gcc -O3 -ffast-math -fopenmp, targeting x86-64-v3 (AVX2/FMA) so the library
-runs on any current x86 host, including the GPU box's.
+runs on any current x86 host, including the one that holds the GPU.
 """
 from __future__ import annotations
 

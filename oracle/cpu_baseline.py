@@ -1,7 +1,7 @@
 """ctypes wrapper of oracle/libvpm_cpu.so -- the CPU baseline (test infrastructure).
 
 Used only by tests (to cross-check the NumPy oracle) and by bench.py's
-cpu_baseline leg (timed on the host cores of the GPU box).
+cpu_baseline leg (timed on the host cores of the machine that holds the GPU).
 """
 from __future__ import annotations
 
