@@ -36,6 +36,10 @@
 #include "vpm_common.h"
 #include "vpm_prepass.h"
 
+#ifndef VPM_FAR64_UNROLL
+#define VPM_FAR64_UNROLL 2
+#endif
+
 namespace vpm {
 
 namespace {
@@ -351,9 +355,14 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
 // p = a_i x x_i per target.
 constexpr double kFarGap = 4.0;
 
+template <typename Real>
+struct Mom {  // a source's moment a_j x x_j
+  Real x, y, z;
+};
+
 template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const Real* p,
-                                            const SrcR<Real, UNIF>& s, const float4& m,
+                                            const SrcR<Real, UNIF>& s, const Mom<Real>& m,
                                             const Real* slo, const PairConst<Real>& k) {
   using S = Slots<OUT>;
   using M = Arith<Real>;
@@ -455,11 +464,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // they sum G in FP64 only.  With velocity and a uniform radius every wave
   // runs the far-field moment form (pair_moment) on distant tiles.
   constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
-  constexpr bool kMoment = S::str && S::vel && UNIF && std::is_same<Real, float>::value;
+  constexpr bool kMoment = S::str && S::vel && UNIF;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
-  __shared__ float lds_mz[kMoment ? TILE : 1];  // a_j x x_j: (p0, p1) of the record, z here
+  __shared__ Real lds_mz[kMoment ? TILE : 1];  // a_j x x_j: (p0, p1) of the record, z here
   __shared__ double lds_x64[kMoment ? BLOCK * R : 1][3];  // the targets' x_i in FP64
   __shared__ double red[6][BLOCK / 64];
   __shared__ double centre[3];
@@ -611,9 +620,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     // Sources per unrolled step: deep unrolling lets the compiler prefetch LDS
     // records for 8 sources, which the heaviest output sets pay for in occupancy.
     constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
+    // FP64 records are twice the registers: the FP64 pass's far loop
+    // prefetches fewer of them.
+    constexpr int kUnrollFar = std::is_same<Real, float>::value ? kUnroll : VPM_FAR64_UNROLL;
     for (int t0 = lo; t0 < hi; t0 += TILE) {
       __syncthreads();
-      float bmn[3] = {INFINITY, INFINITY, INFINITY}, bmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+      Real bmn[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
+      Real bmx[3] = {Real(-INFINITY), Real(-INFINITY), Real(-INFINITY)};
 #pragma unroll
       for (int k = tid; k < TILE; k += BLOCK) {
         const int j = t0 + k;
@@ -630,10 +643,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           if constexpr (kMoment) {
             s.p0 = static_cast<Real>(g.ay * z - g.az * y);
             s.p1 = static_cast<Real>(g.az * x - g.ax * z);
-            lds_mz[k] = static_cast<float>(g.ax * y - g.ay * x);
-            bmn[0] = fminf(bmn[0], s.x); bmx[0] = fmaxf(bmx[0], s.x);
-            bmn[1] = fminf(bmn[1], s.y); bmx[1] = fmaxf(bmx[1], s.y);
-            bmn[2] = fminf(bmn[2], s.z); bmx[2] = fmaxf(bmx[2], s.z);
+            lds_mz[k] = static_cast<Real>(g.ax * y - g.ay * x);
+            bmn[0] = s.x < bmn[0] ? s.x : bmn[0]; bmx[0] = s.x > bmx[0] ? s.x : bmx[0];
+            bmn[1] = s.y < bmn[1] ? s.y : bmn[1]; bmx[1] = s.y > bmx[1] ? s.y : bmx[1];
+            bmn[2] = s.z < bmn[2] ? s.z : bmn[2]; bmx[2] = s.z > bmx[2] ? s.z : bmx[2];
           }
           if constexpr (kStageA64) {
             lds_a64[k][0] = g.ax;
@@ -660,7 +673,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           if constexpr (kStageLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           if constexpr (kMoment) {
             s.p0 = s.p1 = Real(0);
-            lds_mz[k] = 0.0f;
+            lds_mz[k] = Real(0);
           }
           if constexpr (!UNIF) {
             s.hs = Real(0.5);
@@ -674,7 +687,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
         const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const float l = wave_min(bmn[c]), h = wave_max(bmx[c]);
+          const Real l = wave_min(bmn[c]), h = wave_max(bmx[c]);
           if (lane == 0) {
             red[c][wave] = static_cast<double>(l);
             red[3 + c][wave] = static_cast<double>(h);
@@ -706,16 +719,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       // pair is zero throughout such a tile, so its update is skipped.
       auto fold_sub = [&](auto far_tile) {
         constexpr bool FAR = decltype(far_tile)::value;
+        // G leaves its partials every sub-tile also in the FP64 pass when the
+        // far form is on: a far sub-tile's fold (x_i x G + H) must see only
+        // its own G.
+        if constexpr (S::str && (kFold32 || kMoment)) {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {  // zero in FP64-G waves' near sub-tiles
+              gsum[r][c] += static_cast<double>(part[r][kG + c]);
+              part[r][kG + c] = Real(0);
+            }
+        }
         if constexpr (kFold32) {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            if constexpr (S::str) {  // FP32 partials of G (zero in FP64-G waves)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                gsum[r][c] += static_cast<double>(part[r][kG + c]);
-                part[r][kG + c] = Real(0);
-              }
-            }
 #pragma unroll
             for (int q = 0; q < kNAcc; ++q) {
               if (!S::used(q) || (FAR && q >= kU && q < kU + 3)) continue;
@@ -736,10 +754,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       if (far) {  // constant false unless kFar: a far tile sweeps its sub-tiles alone
 #pragma unroll 1
         for (int sb = 0; sb < TILE; sb += SUB) {
-#pragma unroll kUnroll
+#pragma unroll kUnrollFar
           for (int q = 0; q < SUB; ++q) {
             const SrcR<Real, UNIF> s = lds[sb + q];
-            const float4 m = make_float4(s.p0, s.p1, lds_mz[sb + q], 0.0f);
+            const Mom<Real> m{s.p0, s.p1, lds_mz[sb + q]};
             Real slo[3] = {Real(0), Real(0), Real(0)};
             if constexpr (S::pse && kStageLo) {
               const float4 l = lds_lo[sb + q];
@@ -846,7 +864,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
-      for (int q = 0; q < kNAcc; ++q) tot[r][q] = static_cast<double>(part[r][q]);
+      for (int q = 0; q < kNAcc; ++q) tot[r][q] += static_cast<double>(part[r][q]);
       if constexpr (S::str) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) gsum[r][c] += static_cast<double>(part[r][kG + c]);

@@ -338,3 +338,17 @@ def test_sparse_cloud_keeps_the_self_pair_out_of_g(gpu, outputs):
                             sigma_mode=mode, g64="never")
             check({k: res[k] for k in ("vel", "stretch")}, ref, F32C_TOL,
                   f"sparse n={n} {mode}")
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", F64_TOL), ("f32c", F32C_TOL)])
+def test_single_pass_on_ring_crosses_near_and_far_tiles(gpu, precision, tol):
+    """One pass over all source tiles (no source split): every workgroup meets
+    far tiles (moment form) after near ones, so each far sub-tile's fold must
+    see only its own alpha_j sum, in the FP64 pass as in the FP32 one."""
+    x, a, s, nu = ring_set(64)
+    ref = cached("ring64", lambda: oracle_all(x, a, s, nu))
+    outputs = ("vel", "stretch", "pse")
+    print(f"[single-pass ring] n={len(x)} {precision}", flush=True)
+    res = P.forward(*to_dev(x, a, s, nu), outputs=outputs, precision=precision,
+                    sigma_mode="uniform", max_splits=1)
+    check(res, {k: ref[k] for k in outputs}, tol, ("single-pass ring", precision))
