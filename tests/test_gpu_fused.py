@@ -340,15 +340,22 @@ def test_sparse_cloud_keeps_the_self_pair_out_of_g(gpu, outputs):
                   f"sparse n={n} {mode}")
 
 
+@pytest.mark.parametrize("layout", ["ring", "cloud"])
+@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse"),
+                                     ("vel", "stretch", "pse", "vort", "enstrophy")])
 @pytest.mark.parametrize("precision,tol", [("f64", F64_TOL), ("f32c", F32C_TOL)])
-def test_single_pass_on_ring_crosses_near_and_far_tiles(gpu, precision, tol):
+def test_single_pass_crosses_near_and_far_tiles(gpu, precision, tol, outputs, layout):
     """One pass over all source tiles (no source split): every workgroup meets
     far tiles (moment form) after near ones, so each far sub-tile's fold must
-    see only its own alpha_j sum, in the FP64 pass as in the FP32 one."""
-    x, a, s, nu = ring_set(64)
-    ref = cached("ring64", lambda: oracle_all(x, a, s, nu))
-    outputs = ("vel", "stretch", "pse")
-    print(f"[single-pass ring] n={len(x)} {precision}", flush=True)
+    see only its own alpha_j sum, in the FP64 pass as in the FP32 one, in
+    coherent (ring) and unrelated (cloud) waves."""
+    if layout == "ring":
+        x, a, s, nu = ring_set(64)
+    else:
+        x, a, s, _ = FM.random_cloud(4096, seed=41)
+        nu = np.full(len(x), 1e-3)
+    ref = cached(f"{layout}-single", lambda: oracle_all(x, a, s, nu))
+    print(f"[single-pass] {layout} n={len(x)} {precision} {outputs}", flush=True)
     res = P.forward(*to_dev(x, a, s, nu), outputs=outputs, precision=precision,
                     sigma_mode="uniform", max_splits=1)
-    check(res, {k: ref[k] for k in outputs}, tol, ("single-pass ring", precision))
+    check(res, {k: ref[k] for k in outputs}, tol, ("single-pass", layout, precision))
