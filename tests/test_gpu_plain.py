@@ -43,6 +43,7 @@ CASES = [(1, 0), (2, 1), (7, 2), (257, 3), (1000, 4)]
 def _variant_reset():
     yield
     H.VARIANT = "cuda"
+    H.PRECISION = "plain"
 
 
 @pytest.mark.parametrize("variant", ["cuda", "cpu"])
@@ -119,3 +120,21 @@ def test_determinism(gpu):
     u1 = H.get_selfinduced_velocity(x, a, s)
     u2 = H.get_selfinduced_velocity(x, a, s)
     assert np.array_equal(u1, u2)
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", VEC_TOL), ("f32c", 1e-6)])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_fused_precision_switch(gpu, precision, tol, uniform):
+    """PRECISION = "f64" / "f32c" routes velocity, stretching, PSE and
+    enstrophy through the fused pass; the results stay within its bar of the
+    oracle, and the plain-only functions are unaffected."""
+    x, a, s, nu = random_set(1500, 9, uniform_sigma=uniform)
+    H.PRECISION = precision
+    assert rel_l2(H.get_selfinduced_velocity(x, a, s), O.selfinduced_velocity(x, a, s)) <= tol
+    assert rel_l2(H.get_strength_gradients(x, a, s), O.strength_gradients(x, a, s)) <= tol
+    assert rel_l2(H.get_strength_gradients_PSE(x, a, s, nu),
+                  O.strength_gradients_pse(x, a, s, nu)) <= tol
+    ens = H.get_total_enstrophy(x, a, s)
+    assert abs(ens - O.total_enstrophy(x, a, s)) <= 10 * tol * abs(O.total_enstrophy(x, a, s))
+    assert rel_l2(H.get_induced_velocity_at(x[:50], x, a, s),
+                  O.induced_velocity_at(x[:50], x, a, s)) <= VEC_TOL

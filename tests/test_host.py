@@ -127,3 +127,23 @@ def test_no_silent_cpu_fallback():
         ParticleSystem(processing_unit="GPU")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         H.get_selfinduced_velocity(np.zeros((2, 3)), np.zeros((2, 3)), np.ones(2))
+
+
+def test_drop_in_precision_switch_checks():
+    """The drop-in module's fused-pass switch rejects unknown values and the
+    CPU plugin's arithmetic (the fused pass computes the GPU plugin's)."""
+    from openonda_amd import ParticlesPhysicsHip as H
+
+    try:
+        H.PRECISION = "plain"
+        assert H._fused() is None
+        H.PRECISION = "f32c"
+        assert H._fused() == "f32c"
+        H.PRECISION = "f16"
+        with pytest.raises(ValueError, match="PRECISION"):
+            H._fused()
+        H.PRECISION, H.VARIANT = "f64", "cpu"
+        with pytest.raises(ValueError, match="VARIANT"):
+            H._fused()
+    finally:
+        H.PRECISION, H.VARIANT = "plain", "cuda"
