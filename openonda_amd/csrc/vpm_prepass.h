@@ -51,13 +51,29 @@ __global__ __launch_bounds__(kPreBlock) void bbox_partial_kernel(const double* _
   }
 }
 
+// One wave: each lane folds a strided share of the nb partial boxes, then
+// the wave reduces (min/max are exact in any order).
 __global__ __launch_bounds__(64) void bbox_final_kernel(const double* __restrict__ part, int nb,
                                                         double* __restrict__ bbox) {
-  if (threadIdx.x < 6) {
-    double v = part[threadIdx.x];
-    for (int b = 1; b < nb; ++b)
-      v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x]) : fmax(v, part[b * 6 + threadIdx.x]);
-    bbox[threadIdx.x] = v;
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nb; b += 64) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = fmin(mn[c], part[b * 6 + c]);
+      mx[c] = fmax(mx[c], part[b * 6 + 3 + c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    mn[c] = wave_min(mn[c]);
+    mx[c] = wave_max(mx[c]);
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bbox[c] = mn[c];
+      bbox[3 + c] = mx[c];
+    }
   }
 }
 
