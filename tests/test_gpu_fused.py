@@ -127,14 +127,17 @@ def test_cloud_f32c_65536_subsample(gpu):
     assert rel_l2(u, u_ref) <= F32C_TOL
 
 
-def test_presorted_equals_sorted(gpu):
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-13), ("f32c", 2e-6)])
+def test_presorted_equals_sorted(gpu, precision, tol):
+    """presorted=True skips the Morton sort (the caller's order: station by
+    station along the ring, compact blocks, near and far tiles)."""
     x, a, s, nu = ring_set(16)
     d = to_dev(x, a, s, nu)
-    r1 = P.forward(*d, outputs=("vel", "stretch"), precision="f64", sigma_mode="uniform")
-    r2 = P.forward(*d, outputs=("vel", "stretch"), precision="f64", sigma_mode="uniform",
-                   presorted=True)
+    outs = ("vel", "stretch", "pse")
+    r1 = P.forward(*d, outputs=outs, precision=precision, sigma_mode="uniform")
+    r2 = P.forward(*d, outputs=outs, precision=precision, sigma_mode="uniform", presorted=True)
     for k in r1:
-        assert rel_l2(P.from_soa(r1[k]), P.from_soa(r2[k])) <= 1e-13
+        assert rel_l2(P.from_soa(r1[k]), P.from_soa(r2[k])) <= tol, k
 
 
 def test_empty_forward(gpu):
