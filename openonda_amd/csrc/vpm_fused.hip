@@ -36,10 +36,6 @@
 #include "vpm_common.h"
 #include "vpm_prepass.h"
 
-#ifndef VPM_FAR64_UNROLL
-#define VPM_FAR64_UNROLL 2
-#endif
-
 namespace vpm {
 
 namespace {
@@ -622,7 +618,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     constexpr int kUnroll = (S::pse && (S::vort || S::ens)) ? 1 : 8;
     // FP64 records are twice the registers: the FP64 pass's far loop
     // prefetches fewer of them.
-    constexpr int kUnrollFar = std::is_same<Real, float>::value ? kUnroll : VPM_FAR64_UNROLL;
+    constexpr int kUnrollFar = std::is_same<Real, float>::value ? kUnroll : 2;
     for (int t0 = lo; t0 < hi; t0 += TILE) {
       __syncthreads();
       Real bmn[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
