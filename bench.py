@@ -8,9 +8,11 @@ all N^2 pairs (Morton sort + pack + pair pass), inputs resident in HBM.
 Prints one JSON line: pairs/s, the fused kernel's roofline position
 (algorithmic FLOP/s against the MI355X FP32 vector peak, from HIP events on
 the launch stream), rel-L2 of the FP32 result against FP64 on a strided target
-sample, the plain FP64 drop-in kernels' time on the same input, the adjoint /
+sample, the plain FP64 drop-in kernels' time on the same input, the same pass
+called host-to-host (NumPy in and out, PCIe copies included), the adjoint /
 forward time ratio and the CPU baseline (C/OpenMP FP64 port of the reference's
-Numba CPU plugin) timed on this host.
+Numba CPU plugin) timed on this host.  --skip-plain skips the comparison legs
+(plain FP64, fused FP64, host-to-host).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
        [--skip-cpu] [--skip-plain] [--skip-adjoint] [--json-only]
@@ -206,6 +208,43 @@ def main(argv=None):
                                "fp64_tflops": FLOPS_PER_PAIR_VEL_STRETCH * pairs / f64_ms / 1e9,
                                "fp64_peak_tflops": FP64_VECTOR_PEAK_TFLOPS}
         log(f"[bench] fused f64: {f64_ms:.1f} ms")
+
+        # The same pass from host arrays: the drop-in module's boundary takes
+        # (N,3) float64 NumPy arrays and returns NumPy, so each call copies its
+        # inputs over PCIe, runs the pass and copies the result back
+        # (ParticlesPhysicsCuda.py:16-34 does the same).  Wall-clock times.
+        from openonda_amd import ParticlesPhysicsHip as PPH
+
+        log("[bench] host-to-host: one velocity + stretching pass from NumPy")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hp, ha = P.to_soa(x, dev), P.to_soa(a, dev)
+        hs = torch.from_numpy(s).to(dev)
+        ho = P.forward(hp, ha, hs, outputs=outputs, precision="f32c", sigma_mode="uniform")
+        hu, hS = P.from_soa(ho["vel"]), P.from_soa(ho["stretch"])
+        one_s = time.perf_counter() - t0
+        del hp, ha, hs, ho
+        log("[bench] host-to-host: drop-in get_selfinduced_velocity + get_strength_gradients")
+        prec0 = PPH.PRECISION
+        PPH.PRECISION = "f32c"
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            du = PPH.get_selfinduced_velocity(x, a, s)
+            dS = PPH.get_strength_gradients(x, a, s)
+            two_s = time.perf_counter() - t0
+        finally:
+            PPH.PRECISION = prec0
+        agree = max(float(np.linalg.norm(du - hu) / np.linalg.norm(hu)),
+                    float(np.linalg.norm(dS - hS) / np.linalg.norm(hS)))
+        result["host_to_host"] = {
+            "one_pass_ms": one_s * 1e3, "one_pass_pairs_per_s": pairs / one_s,
+            "drop_in_two_calls_ms": two_s * 1e3, "drop_in_pairs_per_s": pairs / two_s,
+            "drop_in_vs_one_pass_rel_l2": agree,
+            "sample": "wall clock of one call each, NumPy in and out (PCIe copies, "
+                      "(N,3)<->(3,N) transposes, Morton sort, pass); not the headline value"}
+        log(f"[bench] host-to-host: one pass {one_s * 1e3:.1f} ms, drop-in pair "
+            f"{two_s * 1e3:.1f} ms (rel {agree:.1e})")
 
     # ---- CPU baseline + FP64 accuracy sample -------------------------------
     if not args.skip_cpu:

@@ -54,6 +54,9 @@ def test_batch_forward_matches_per_set(gpu, precision, tol):
         tx, ta = P.to_soa(x), P.to_soa(a)
         ts, tn = torch.from_numpy(sg).cuda(), torch.from_numpy(nu).cuda()
         if len(x) < 2:
+            # a lone particle induces nothing on itself (self pair excluded)
+            for t in (bs.vel, bs.str_, bs.pse_):
+                assert torch.count_nonzero(t[:, sl]) == 0
             continue
         print(f"[batch] {precision} set {k} n={len(x)}", flush=True)
         assert rel(bs.vel[:, sl], P.selfinduced_velocity(tx, ta, ts)) <= tol
