@@ -330,6 +330,27 @@ __device__ __forceinline__ void pair_update(Real* acc, double* g64, const Tgt<Re
   }
 }
 
+// g(0) of the self pair, by the operations pair_update applies to it (d = 0
+// gives A = 1 exactly): the product g(0) alpha_i formed from it is the term
+// the FP64 sums of G took for the self pair.
+template <typename Real, bool UNIF>
+__device__ __forceinline__ Real self_g(Real hs, const PairConst<Real>& k) {
+  using M = Arith<Real>;
+  Real A = Real(1);
+  asm volatile("" : "+v"(A));  // evaluated at run time, as in the sweep
+  const Real t1 = M::rsq(A);
+  const Real t2 = t1 * t1;
+  const Real t3 = t2 * t1;
+  const Real t5 = t3 * t2;
+  Real g = M::fma(k.c15, t5, t3);
+  if constexpr (!UNIF) {
+    const Real inv = M::rcp(hs + hs);
+    const Real inv2 = inv * inv;
+    g *= inv2 * inv;
+  }
+  return g;
+}
+
 // Far-field moment form of velocity + stretching (+ PSE), FP32, uniform
 // radius.  With d = x_i - x_j and coordinates relative to the workgroup
 // centre,
@@ -659,7 +680,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
             s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
           }
         } else {  // padding: far away, zero strength -> contributes exactly 0
-          s.x = s.y = s.z = static_cast<Real>(1e18);
+          // (PSE weights alpha_j - alpha_i = -alpha_i by t^9: at this distance
+          // t^9 underflows to 0 in either precision; A stays finite)
+          s.x = s.y = s.z = static_cast<Real>(std::is_same<Real, float>::value ? 1e18 : 1e40);
           s.ax = s.ay = s.az = Real(0);
           if constexpr (kStageA64) {
             lds_a64[k][0] = 0.0;
@@ -864,6 +887,28 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       if constexpr (S::str) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) gsum[r][c] += static_cast<double>(part[r][kG + c]);
+      }
+    }
+  }
+  // The FP64 sums of G (coherent waves of an FP32 pass, the FP64 pass) hold
+  // the self pair's g(0) alpha_i; the FP32 partials leave it out (pair_update,
+  // SELF).  The reference sums the stretching over j != i, and alpha_i x G
+  // cancels the term only to a rounding of g(0) |alpha_i|^2 (a lone particle
+  // came out at 5e-15, not 0), so it is subtracted here, as the same rounded
+  // product the sweep added, for the targets whose index lies in [lo, hi).
+  if constexpr (S::str) {
+    if (!kFold32 || g64) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int k = base + r * BLOCK + tid;
+        if (!valid[r] || k < lo || k >= hi) continue;
+        const double g0 = static_cast<double>(self_g<Real, UNIF>(t[r].hs, kc));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double term = g0 * a64t[r][c];
+          asm volatile("" : "+v"(term));  // rounded, not fused into the subtraction
+          gsum[r][c] -= term;
+        }
       }
     }
   }

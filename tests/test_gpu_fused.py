@@ -148,6 +148,26 @@ def test_empty_forward(gpu):
     assert float(res["enstrophy"].item()) == 0.0
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+@pytest.mark.parametrize("sigma_mode", ["uniform", "general"])
+def test_one_particle_forward(gpu, precision, sigma_mode):
+    """A lone particle: the reference sums over j != i, so velocity,
+    stretching and PSE are exactly zero; vorticity and enstrophy keep the
+    i = j term."""
+    x = np.array([[0.3, -1.2, 2.5]])
+    a = np.array([[0.7, -0.4, 1.1]])
+    s, nu = np.array([0.17]), np.array([2e-3])
+    print(f"[fused] one particle {precision} {sigma_mode}", flush=True)
+    res = P.forward(*to_dev(x, a, s, nu), outputs=ALL, precision=precision,
+                    sigma_mode=sigma_mode, max_splits=1)
+    for k in ("vel", "stretch", "pse"):
+        assert torch.count_nonzero(res[k]) == 0, (k, res[k])
+    ref = oracle_all(x, a, s, nu)
+    tol = F64_TOL if precision == "f64" else F32C_TOL
+    assert rel_l2(P.from_soa(res["vort"]), ref["vort"]) <= tol
+    assert float(res["enstrophy"].item()) == pytest.approx(ref["enstrophy"], rel=10 * tol)
+
+
 def test_bad_arguments(gpu):
     x, a, s, nu = to_dev(*ring_set(4))
     with pytest.raises(ValueError):
