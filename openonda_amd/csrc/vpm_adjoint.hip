@@ -402,9 +402,11 @@ __global__ __launch_bounds__(BLOCK) void adjoint_kernel(AdjArgs a) {
         s.zy = static_cast<Real>(A.z[N + j]);
         s.zz = static_cast<Real>(A.z[2 * N + j]);
         s.hs = static_cast<Real>(0.5 * A.sig[j]);
-      } else {  // far away, all data zero: every term vanishes
+      } else {  // far away, all data zero: every term vanishes (PSE's -k_ij
+                // lambda_i too: at this distance its powers of t underflow to 0
+                // in either precision)
         s = AdjSrc<Real>{};
-        s.x = s.y = s.z = static_cast<Real>(1e18);
+        s.x = s.y = s.z = static_cast<Real>(std::is_same<Real, float>::value ? 1e18 : 1e40);
         s.hs = Real(0.5);
       }
       s.pad = Real(0);
