@@ -168,6 +168,28 @@ def test_one_particle_forward(gpu, precision, sigma_mode):
     assert float(res["enstrophy"].item()) == pytest.approx(ref["enstrophy"], rel=10 * tol)
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", F64_TOL), ("f32c", F32C_TOL)])
+@pytest.mark.parametrize("sigma_mode", ["uniform", "general"])
+def test_coincident_particles(gpu, precision, tol, sigma_mode):
+    """Distinct particles at one position: the reference sums over j != i, so
+    such pairs count (d = 0: nothing in the velocity or the stretching's
+    r-term, but g(0) alpha_i x alpha_j in the stretching and the full PSE
+    weight; the vorticity's r^2 > 1e-8 test drops them).  Only the particle's
+    own index is left out, not its position."""
+    rng = np.random.default_rng(21)
+    n0, nd = 300, 40
+    x = rng.uniform(-1.0, 1.0, (n0, 3))
+    x = np.concatenate([x, x[rng.choice(n0, nd, replace=False)]])
+    n = len(x)
+    a = rng.normal(0.0, 1.0, (n, 3)) / n
+    s = np.full(n, 0.15) if sigma_mode == "uniform" else rng.uniform(0.1, 0.2, n)
+    nu = rng.uniform(1e-3, 1e-2, n)
+    print(f"[fused] coincident {precision} {sigma_mode}", flush=True)
+    res = P.forward(*to_dev(x, a, s, nu), outputs=ALL, precision=precision,
+                    sigma_mode=sigma_mode)
+    check(res, oracle_all(x, a, s, nu), tol, ("coincident", precision, sigma_mode))
+
+
 def test_bad_arguments(gpu):
     x, a, s, nu = to_dev(*ring_set(4))
     with pytest.raises(ValueError):
