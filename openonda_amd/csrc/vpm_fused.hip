@@ -132,7 +132,7 @@ struct Slots {
   // FP32 partials kG..kG+2 into it per sub-tile.
   static __host__ __device__ constexpr bool used(int k) {
     return (k < 3 && vel) || (k >= 6 && k < 9 && str) || (k >= 9 && k < 12 && pse) ||
-           (k >= 12 && k < 15 && vort) || (k >= 15 && ens);
+           (k >= 12 && k < 15 && vort) || (k >= 15 && k < 18 && ens);
   }
   // position of slot k among the used ones (the split partial layout)
   static __host__ __device__ constexpr int packed(int k) {
