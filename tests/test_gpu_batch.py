@@ -66,6 +66,37 @@ def test_batch_forward_matches_per_set(gpu, precision, tol):
         assert float(bs.enstrophy[k]) == pytest.approx(O.total_enstrophy(x, a, sg), rel=10 * tol)
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32c", 1e-6)])
+def test_batch_coincident_particles(gpu, precision, tol):
+    """Sets with distinct particles at one position (a coincident pair, and a
+    cloud with 40 duplicated positions): the pairs count as in the
+    reference's j != i sums, only each particle's own index is left out."""
+    rng = np.random.default_rng(31)
+    x0 = rng.uniform(-1, 1, (300, 3))
+    xc = np.concatenate([x0, x0[rng.choice(300, 40, replace=False)]])
+    sets = []
+    for x in (np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3]]), rng.uniform(-1, 1, (3, 3)), xc):
+        n = len(x)
+        a = rng.normal(0, 1, (n, 3)) / n
+        sets.append((x, a, np.full(n, 0.15), np.full(n, 2e-3)))
+    bs = B.BatchedSystem(sets, dt=0.01, precision=precision)
+    bs.evaluate(bs.x, bs.a, 31, True, bs.stream.cuda_stream)
+    bs.synchronize()
+    for k, (x, a, sg, nu) in enumerate(sets):
+        print(f"[batch] coincident {precision} set {k} n={len(x)}", flush=True)
+        sl = bs.set_slice(k)
+        tx, ta = P.to_soa(x), P.to_soa(a)
+        ts, tn = torch.from_numpy(sg).cuda(), torch.from_numpy(nu).cuda()
+        for got, ref in ((bs.vel[:, sl], P.selfinduced_velocity(tx, ta, ts)),
+                         (bs.str_[:, sl], P.strength_gradients(tx, ta, ts)),
+                         (bs.pse_[:, sl], P.strength_gradients_pse(tx, ta, ts, tn))):
+            if torch.count_nonzero(ref) == 0:  # the coincident pair's velocity
+                assert torch.count_nonzero(got) == 0
+            else:
+                assert rel(got, ref) <= tol
+        assert float(bs.enstrophy[k]) == pytest.approx(O.total_enstrophy(x, a, sg), rel=10 * tol)
+
+
 @pytest.mark.parametrize("cfg", range(10))
 def test_batch_tile_configs(gpu, cfg):
     sets = small_sets(1)
