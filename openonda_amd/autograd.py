@@ -28,6 +28,16 @@ from . import physics as P
 _ALL = ("vel", "stretch", "pse")
 
 
+def _resolve_sigma_mode(sigma, sigma_mode):
+    """"auto" -> "uniform" when every radius equals sigma[0], else "general"
+    (one device check); "uniform" and "general" pass through."""
+    if sigma_mode == "auto":
+        return "uniform" if P.sigma_is_uniform(sigma) else "general"
+    if sigma_mode not in ("uniform", "general"):
+        raise ValueError("sigma_mode must be 'auto', 'uniform' or 'general'")
+    return sigma_mode
+
+
 def _opts(precision, sigma_mode, presorted=False, max_splits=0):
     return L.ForwardOpts(0, P.PRECISIONS[precision],
                          L.SIGMA_UNIFORM if sigma_mode == "uniform" else L.SIGMA_GENERAL,
@@ -35,11 +45,14 @@ def _opts(precision, sigma_mode, presorted=False, max_splits=0):
 
 
 def adjoint(pos, alpha, sigma, nu, lam=None, mu=None, xi=None, precision="f32c",
-            sigma_mode="uniform", max_splits=0):
+            sigma_mode="auto", max_splits=0):
     """dL/dx, dL/dalpha for L = <lam, u> + <mu, S> + <xi, D> (each (3,N) or None).
 
+    ``sigma_mode``: "auto" checks the radii on the device, "uniform" asserts
+    sigma_i == sigma_0 (unchecked), "general" uses sigma_ij per pair.
     ``max_splits`` caps the source splits used when the targets alone cannot
     fill the device (0: automatic, 1: never split)."""
+    sigma_mode = _resolve_sigma_mode(sigma, sigma_mode)
     p = P._particles(pos, alpha, sigma, nu if xi is not None else None)
     n = p.n
     cot = [t.contiguous() if t is not None else None for t in (lam, mu, xi)]
@@ -67,6 +80,7 @@ class PairSums(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, pos, alpha, sigma, nu, outputs, precision, sigma_mode):
+        sigma_mode = _resolve_sigma_mode(sigma, sigma_mode)
         res = P.forward(pos.detach(), alpha.detach(), sigma, nu if "pse" in outputs else None,
                         outputs=outputs, precision=precision, sigma_mode=sigma_mode)
         ctx.save_for_backward(pos, alpha, sigma, nu)
@@ -82,7 +96,9 @@ class PairSums(torch.autograd.Function):
         return dpos, dstr, None, None, None, None, None
 
 
-def pair_sums(pos, alpha, sigma, nu=None, outputs=_ALL, precision="f32c", sigma_mode="uniform"):
+def pair_sums(pos, alpha, sigma, nu=None, outputs=_ALL, precision="f32c", sigma_mode="auto"):
+    """Differentiable (vel, stretch, pse) of (pos, alpha); ``sigma_mode`` as in
+    :func:`adjoint`."""
     if nu is None:
         nu = torch.zeros_like(sigma)
     return PairSums.apply(pos, alpha, sigma, nu, tuple(outputs), precision, sigma_mode)
@@ -115,9 +131,11 @@ def probe_velocity(targets, pos, alpha, sigma):
     return ProbeVelocity.apply(targets, pos, alpha, sigma)
 
 
-def rk4_coupled_step(pos, alpha, sigma, nu, dt, precision="f32c", sigma_mode="uniform",
+def rk4_coupled_step(pos, alpha, sigma, nu, dt, precision="f32c", sigma_mode="auto",
                      pse=True):
-    """One classical RK4 step of the coupled particle ODE."""
+    """One classical RK4 step of the coupled particle ODE (the radii are
+    constant through it: "auto" is resolved once)."""
+    sigma_mode = _resolve_sigma_mode(sigma, sigma_mode)
     outs = ("vel", "stretch", "pse") if pse else ("vel", "stretch")
 
     def rhs(x, a):
@@ -133,9 +151,10 @@ def rk4_coupled_step(pos, alpha, sigma, nu, dt, precision="f32c", sigma_mode="un
 
 
 def rollout(pos, alpha, sigma, nu, dt, steps, checkpoint_every=10, precision="f32c",
-            sigma_mode="uniform", pse=True):
+            sigma_mode="auto", pse=True):
     """``steps`` RK4 steps; with checkpoint_every > 0 each segment is
     recomputed in the backward pass instead of storing its stage tensors."""
+    sigma_mode = _resolve_sigma_mode(sigma, sigma_mode)
     x, a = pos, alpha
     k = 0
     while k < steps:

@@ -15,6 +15,7 @@ import json
 import math
 import os
 import threading
+import warnings
 from pathlib import Path
 
 import torch
@@ -70,8 +71,9 @@ def _time(fn, reps):
 
 def tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mode="uniform",
          max_rel=1e-6, reps=2, path=None, save=True):
-    """Times every tile configuration on this input; returns the report and
-    (optionally) records the winner in the cache."""
+    """Times every tile configuration on this input; returns (best, report)
+    and (optionally) records the winner in the cache.  When no configuration
+    meets ``max_rel``, returns (None, report), caches nothing and warns."""
     P.require_gpu()
     n = pos.shape[1]
     ref = P.forward(pos, alpha, sigma, nu, outputs=outputs, precision="f64",
@@ -93,7 +95,14 @@ def tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mode="uni
                 d = float((v - ref[k]).norm() / ref[k].norm())
             rel = max(rel, d)
         report.append({"config": cfg, "shape": shape, "ms": ms, "rel_l2": rel})
-    ok = [r for r in report if r["rel_l2"] <= max_rel] or report
+    ok = [r for r in report if r["rel_l2"] <= max_rel]
+    if not ok:
+        # no shape meets the accuracy bound on this input: keep the library's
+        # default shape (nothing is cached, so tile_config="auto" falls back to it)
+        warnings.warn(f"autotune: no tile configuration reached rel-L2 <= {max_rel:g} "
+                      f"(best {min(r['rel_l2'] for r in report):.2e}); nothing cached",
+                      RuntimeWarning, stacklevel=2)
+        return None, report
     best = min(ok, key=lambda r: r["ms"])
     if save:
         with _lock:
@@ -167,7 +176,10 @@ def main(argv=None):
     for r in report:
         print(f"config {r['config']} {tuple(r['shape'])}: {r['ms']:.2f} ms rel-L2 {r['rel_l2']:.2e}",
               flush=True)
-    print(f"best: config {best['config']} ({best['ms']:.2f} ms)", flush=True)
+    if best is None:
+        print("no configuration within the accuracy bound; nothing cached", flush=True)
+    else:
+        print(f"best: config {best['config']} ({best['ms']:.2f} ms)", flush=True)
 
 
 if __name__ == "__main__":

@@ -148,11 +148,13 @@ class ParticleSystem:
         self._hist = None
         self._unif = None
 
-    def _set_state(self, pos, vel, st, sig, nu, nut, gid):
+    def _set_state(self, pos, vel, st, sig, nu, nut, gid, hist=None):
+        """Replaces the particle arrays; ``hist`` is the strength history of the
+        multistep update for the new set (None: none yet)."""
         self._pos, self._vel, self._str = pos.contiguous(), vel.contiguous(), st.contiguous()
         self._sigma, self._nu, self._nut, self._gid = (sig.contiguous(), nu.contiguous(),
                                                        nut.contiguous(), gid.contiguous())
-        self._hist = None  # strength history refers to the old particle set
+        self._hist = None if hist is None else [h.contiguous() for h in hist]
         self._unif = None
 
     @property
@@ -390,13 +392,21 @@ class ParticleSystem:
                                 np.array([int(particle.group_id)], dtype=np.int64))
 
     def remove_particles(self, particle_indices):                  # :395-400
+        """Removes the particles at the given indices (out-of-range ones are
+        ignored, as in the reference).  The strength history of the multistep
+        update keeps the rows of the remaining particles, so the next
+        update_strengths continues the same scheme; the reference keeps its
+        history arrays at the old size, which fails on the next step after a
+        real removal and continues unchanged when nothing was removed."""
         idx = sorted(set(int(i) for i in particle_indices))
         idx = [i for i in idx if 0 <= i < self.n]
-        keep = torch.ones(self.n, dtype=torch.bool, device=self.device)
         if idx:
+            keep = torch.ones(self.n, dtype=torch.bool, device=self.device)
             keep[torch.tensor(idx, device=self.device)] = False
-        self._set_state(self._pos[:, keep], self._vel[:, keep], self._str[:, keep],
-                        self._sigma[keep], self._nu[keep], self._nut[keep], self._gid[keep])
+            hist = None if self._hist is None else [h[:, keep] for h in self._hist]
+            self._set_state(self._pos[:, keep], self._vel[:, keep], self._str[:, keep],
+                            self._sigma[keep], self._nu[keep], self._nut[keep], self._gid[keep],
+                            hist)
         self._log(f"(info) Removed {len(particle_indices)} particles")
 
     def remove_all_particles(self):                                # :403-406
@@ -431,12 +441,20 @@ class ParticleSystem:
             group_id = np.full(N, int(group_id), dtype=np.int64)
         d = self.device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(d)
+        new_str = P.to_soa(strengths, d)
+        # Particles added after the multistep strength update has started have
+        # no history of their own: it is taken as their current strength (a
+        # constant past), so their first multistep update is consistent
+        # instead of scaling the strength by the scheme's leading coefficient.
+        # The reference's history arrays keep the old size (it fails there).
+        hist = None if self._hist is None else [torch.cat([h, new_str], 1) for h in self._hist]
         self._set_state(torch.cat([self._pos, P.to_soa(positions, d)], 1),
                         torch.cat([self._vel, P.to_soa(velocities, d)], 1),
-                        torch.cat([self._str, P.to_soa(strengths, d)], 1),
+                        torch.cat([self._str, new_str], 1),
                         torch.cat([self._sigma, t(radii)]), torch.cat([self._nu, t(viscosities)]),
                         torch.cat([self._nut, t(np.asarray(viscosities_t, dtype=np.float64).reshape(-1))]),
-                        torch.cat([self._gid, t(np.asarray(group_id, dtype=np.int64).reshape(-1))]))
+                        torch.cat([self._gid, t(np.asarray(group_id, dtype=np.int64).reshape(-1))]),
+                        hist)
 
     def load_particle_field_from_backup(self, filename: str):      # :466-490
         pts, d = vtp.read_vtp(filename)

@@ -57,6 +57,11 @@ def _f64(a, shape_tail=None, name="array"):
     return a
 
 
+def _rows(rows, n):
+    """Target rows of a self-evaluation: all n, or the given indices."""
+    return np.arange(n) if rows is None else np.asarray(rows, dtype=np.int64).reshape(-1)
+
+
 def _chunks(m, n, budget=4_000_000):
     step = max(1, budget // max(n, 1))
     for s in range(0, m, step):
@@ -90,30 +95,32 @@ def induced_velocity_at(targets, x, alpha, sigma):
     return out * ONE_OVER_FOUR_PI                                  # PPC:95-97
 
 
-def selfinduced_velocity(x, alpha, sigma, variant="cuda"):
+def selfinduced_velocity(x, alpha, sigma, variant="cuda", rows=None):
     """u_i = -1/(4pi) sum_{j!=i} g(rho^2)/sigma^3 (r_ij x alpha_j).
 
     cuda: sigma = (sigma_i + sigma_j)/2, ParticlesPhysicsCuda.py:133-191.
     cpu : sigma = sigma_i (target radius), ParticlesPhysics.py:81-143 (:115).
+    ``rows``: evaluate only these targets i (default: all; output in that order).
     """
     _check_variant(variant)
     x = _f64(x, (3,), "positions")
     a = _f64(alpha, (3,), "strengths")
     s = _f64(sigma)
     n = x.shape[0]
-    out = np.zeros((n, 3))
-    for b, e in _chunks(n, n):
-        r = x[b:e, None, :] - x[None, :, :]
+    R = _rows(rows, n)
+    out = np.zeros((len(R), 3))
+    for b, e in _chunks(len(R), n):
+        ii = R[b:e]
+        r = x[ii, None, :] - x[None, :, :]
         r2 = r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1] + r[..., 2] * r[..., 2]
         if variant == "cuda":
-            sg = (s[b:e, None] + s[None, :]) / 2.0                 # PPC:166
+            sg = (s[ii, None] + s[None, :]) / 2.0                  # PPC:166
         else:
-            sg = np.broadcast_to(s[b:e, None], r2.shape)          # PPH:115
+            sg = np.broadcast_to(s[ii, None], r2.shape)           # PPH:115
         sg2 = sg * sg
         rho2 = r2 / sg2
         q = (rho2 + 2.5) / ((rho2 + 1.0) ** 2.5 * sg * sg2)       # PPC:177
-        idx = np.arange(b, e)
-        q[idx - b, idx] = 0.0                                      # j != i, PPC:160
+        q[np.arange(e - b), ii] = 0.0                              # j != i, PPC:160
         c = np.cross(r, a[None, :, :])
         out[b:e] = -np.einsum("mn,mnk->mk", q, c)
     return out * ONE_OVER_FOUR_PI
@@ -161,31 +168,33 @@ def vorticity_field_at(targets, x, alpha, sigma, variant="cuda"):
 # ---------------------------------------------------------------------------
 # Stretching (transpose scheme)
 # ---------------------------------------------------------------------------
-def strength_gradients(x, alpha, sigma):
+def strength_gradients(x, alpha, sigma, rows=None):
     """dalpha_i/dt = 1/(4pi) sum_{j!=i} [f1 (a_i x a_j) + f2 (a_i.(r x a_j)) r].
 
     f1 = (A+1.5)/(s^3 A^2.5), f2 = 3(A+2.5)/(s^5 A^3.5), A = r^2/s^2 + 1,
     s = (s_i+s_j)/2.  ParticlesPhysicsCuda.py:375-445 and
-    ParticlesPhysics.py:268-312 (same arithmetic).
+    ParticlesPhysics.py:268-312 (same arithmetic).  ``rows`` as in
+    :func:`selfinduced_velocity`.
     """
     x = _f64(x, (3,), "positions")
     a = _f64(alpha, (3,), "strengths")
     s = _f64(sigma)
     n = x.shape[0]
-    out = np.zeros((n, 3))
-    for b, e in _chunks(n, n):
-        r = x[b:e, None, :] - x[None, :, :]                        # PPC:403-405
+    R = _rows(rows, n)
+    out = np.zeros((len(R), 3))
+    for b, e in _chunks(len(R), n):
+        ii = R[b:e]
+        r = x[ii, None, :] - x[None, :, :]                         # PPC:403-405
         r2 = r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1] + r[..., 2] * r[..., 2]
-        sg = (s[b:e, None] + s[None, :]) * 0.5                      # PPC:409
+        sg = (s[ii, None] + s[None, :]) * 0.5                       # PPC:409
         sg2 = sg * sg
         sg3 = sg2 * sg
         A = r2 / sg2 + 1.0                                          # PPC:413
         f1 = (A + 1.5) / (sg3 * A ** 2.5)                           # PPC:416
         f2 = 3.0 * (A + 2.5) / (sg3 * sg2 * A ** 3.5)               # PPC:417
-        idx = np.arange(b, e)
-        f1[idx - b, idx] = 0.0
-        f2[idx - b, idx] = 0.0
-        ai = a[b:e, None, :]
+        f1[np.arange(e - b), ii] = 0.0
+        f2[np.arange(e - b), ii] = 0.0
+        ai = a[ii, None, :]
         p1 = np.cross(np.broadcast_to(ai, r.shape), a[None, :, :])  # PPC:420-422
         p2 = np.cross(r, a[None, :, :])                              # PPC:425-427
         p3 = np.einsum("mnk,mnk->mn", np.broadcast_to(ai, r.shape), p2)  # PPC:430-434
@@ -196,11 +205,12 @@ def strength_gradients(x, alpha, sigma):
 # ---------------------------------------------------------------------------
 # PSE viscous diffusion
 # ---------------------------------------------------------------------------
-def strength_gradients_pse(x, alpha, sigma, nu, variant="cuda"):
+def strength_gradients_pse(x, alpha, sigma, nu, variant="cuda", rows=None):
     """PSE: dalpha_i/dt = 1/(4pi) sum_{j!=i} 105 nu s^3/(s^5 A^4.5) (a_j - a_i).
 
     cuda: nu = nu_i (target), A = r^2/s^2 + 1, ParticlesPhysicsCuda.py:478-530.
     cpu : nu = (nu_i+nu_j)/2, A = r^2/s^2 + 1 + 1e-8, ParticlesPhysics.py:318-382.
+    ``rows`` as in :func:`selfinduced_velocity`.
     """
     _check_variant(variant)
     x = _f64(x, (3,), "positions")
@@ -208,25 +218,26 @@ def strength_gradients_pse(x, alpha, sigma, nu, variant="cuda"):
     s = _f64(sigma)
     v = _f64(nu)
     n = x.shape[0]
-    out = np.zeros((n, 3))
-    for b, e in _chunks(n, n):
-        r = x[b:e, None, :] - x[None, :, :]
+    R = _rows(rows, n)
+    out = np.zeros((len(R), 3))
+    for b, e in _chunks(len(R), n):
+        ii = R[b:e]
+        r = x[ii, None, :] - x[None, :, :]
         r2 = r[..., 0] * r[..., 0] + r[..., 1] * r[..., 1] + r[..., 2] * r[..., 2]
-        sg = (s[b:e, None] + s[None, :]) * 0.5
+        sg = (s[ii, None] + s[None, :]) * 0.5
         sg2 = sg * sg
         sg3 = sg2 * sg
         sg5 = sg2 * sg3
         if variant == "cuda":
             A = r2 / sg2 + 1.0                                      # PPC:518
-            vis = np.broadcast_to(v[b:e, None], r2.shape)           # PPC:520 (nu_i)
+            vis = np.broadcast_to(v[ii, None], r2.shape)            # PPC:520 (nu_i)
         else:
             A = r2 / sg2 + 1.0 + EPS_PAIR                           # PPH:369
-            vis = (v[b:e, None] + v[None, :]) * 0.5                 # PPH:360
+            vis = (v[ii, None] + v[None, :]) * 0.5                  # PPH:360
         f3 = 105.0 * vis / (sg5 * A ** 4.5)                         # PPC:520
         w = f3 * sg3
-        idx = np.arange(b, e)
-        w[idx - b, idx] = 0.0
-        da = a[None, :, :] - a[b:e, None, :]                        # PPC:523-525
+        w[np.arange(e - b), ii] = 0.0
+        da = a[None, :, :] - a[ii, None, :]                         # PPC:523-525
         out[b:e] = np.einsum("mn,mnk->mk", w, da)
     return out * ONE_OVER_FOUR_PI
 

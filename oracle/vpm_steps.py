@@ -143,3 +143,17 @@ class ReferenceStepper:
         self.update_velocities()
         self.update_strengths()
         self.update_state()
+
+    def remove(self, indices):
+        """Drops the particles at ``indices`` (vpmModule.py:395-400 pops them
+        from the particle list).  The reference leaves its strength-history
+        arrays at the old size, so its next multistep update fails on the
+        shape; here the history keeps the remaining particles' rows, which is
+        the product's defined behaviour.  An empty index list changes nothing,
+        as in the reference."""
+        keep = np.ones(len(self.x), dtype=bool)
+        keep[[i for i in indices if 0 <= i < len(self.x)]] = False
+        for k in ("x", "alpha", "sigma", "nu", "nu_t", "velocity", "prev", "prev2", "prev3"):
+            v = getattr(self, k)
+            if v is not None:
+                setattr(self, k, v[keep].copy())

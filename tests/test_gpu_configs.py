@@ -1,0 +1,237 @@
+"""GPU parity of the five benchmark workloads at their own sizes.
+
+Each workload is compared with FP64 CPU references on the same seeded
+inputs: the NumPy oracle where it finishes in seconds (PSE always: its GPU
+form), otherwise the C/OpenMP FP64 port of the CPU plugin (oracle/vpm_cpu.c,
+itself checked against the NumPy oracle; velocity and stretching coincide
+with the GPU plugin's for a uniform radius) on strided target samples
+against all sources.  The plain FP64
+kernels are held to 1e-12, the FP32-compensated pass to rel-L2 1e-6 per
+output.
+
+1. ring N = 4096, plain FP64 velocity + stretching, one RK4 step (dt = 0.01)
+   of the ParticleSystem against the step oracle;
+2. leapfrogging rings N = 65,536: FP32-compensated velocity + stretching +
+   PSE (nu = 1e-3) at step 0 against the FP64 port, then RK4 steps in FP32c
+   and FP64 from the same state, and the FP32c outputs on the advanced state;
+3. random clouds N = 2^16, 2^18 (FP32c, fused FP64, plain FP64) and 2^20
+   (plain FP64) against the FP64 port;
+4. ring N = 131,072: FP32c velocity + stretching + PSE against the FP64 port,
+   probe velocity against the NumPy oracle;
+5. 96 uneven sets in one persistent launch, before and after graph-replayed
+   RK4 steps with the LES filter (no re-sort in between), every set against
+   the FP64 port and the oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_baseline as CB
+from oracle import vpm_oracle as O
+from oracle.vpm_steps import ReferenceStepper
+from openonda_amd import autograd as AG
+from openonda_amd import batch as B
+from openonda_amd import flow_models as FM
+from openonda_amd import physics as P
+from openonda_amd.vpmModule import ParticleSystem
+
+pytestmark = pytest.mark.gpu
+
+F32C_TOL = 1e-6
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def strided(n, m):
+    return np.arange(0, n, max(1, n // m), dtype=np.int64)[:m]
+
+
+def dev(x, a, s, nu):
+    return (P.to_soa(x), P.to_soa(a), torch.from_numpy(np.ascontiguousarray(s)).cuda(),
+            torch.from_numpy(np.ascontiguousarray(nu)).cuda())
+
+
+def rows(t, idx):
+    """(3, N) device tensor -> (len(idx), 3) NumPy rows at idx."""
+    return t[:, torch.from_numpy(idx).cuda()].t().cpu().numpy()
+
+
+def port(x, a, s, nu, idx, outputs):
+    ref = {}
+    if "vel" in outputs:
+        ref["vel"] = CB.selfinduced_velocity(x, a, s, idx)
+    if "stretch" in outputs:
+        ref["stretch"] = CB.strength_gradients(x, a, s, idx)
+    if "pse" in outputs:
+        # the GPU plugin's PSE (nu_i, A = rho^2 + 1) from the NumPy oracle: the
+        # port restates the CPU plugin's form (nu_ij, A + 1e-8)
+        ref["pse"] = O.strength_gradients_pse(x, a, s, nu, "cuda", rows=idx)
+    return ref
+
+
+# ------------------------------------------------------------------ config 1
+def test_config1_ring_plain_rk4_step(gpu):
+    x, a, s, nu = FM.torus_ring(64)
+    assert len(x) == 4096
+    print("[config1] ring N=4096 plain FP64 RK4 step", flush=True)
+    ref = ReferenceStepper(x, a, s, nu, dt=0.01, method="RK4", flow_model="DNS",
+                           viscous_scheme="CoreSpreading", variant="cuda")
+    ps = ParticleSystem(time_step_size=0.01, time_integration_method="RK4", processing_unit="GPU",
+                        flow_model="DNS", viscous_scheme="CoreSpreading", precision="plain",
+                        write_backups=False)
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
+    ref.step()
+    ps.update_velocities()
+    ps.update_strengths()
+    ps.update_state()
+    assert rel(ps.get_particle_velocities(), ref.velocity) <= 1e-12
+    assert rel(ps.get_particle_strengths(), ref.alpha) <= 1e-12
+    assert rel(ps.get_particle_positions(), ref.x) <= 1e-13
+
+
+# ------------------------------------------------------------------ config 2
+@pytest.fixture(scope="module")
+def leapfrog():
+    x, a, s, nu = FM.leapfrog_rings()
+    assert len(x) == 65536
+    return x, a, s, nu
+
+
+def test_config2_leapfrog_step0_against_fp64(gpu, leapfrog):
+    x, a, s, nu = leapfrog
+    print("[config2] leapfrog N=65536 f32c vel+stretch+pse step 0", flush=True)
+    outs = ("vel", "stretch", "pse")
+    res = P.forward(*dev(x, a, s, nu), outputs=outs, precision="f32c")
+    idx = strided(len(x), 512)
+    ref = port(x, a, s, nu, idx, outs)
+    for k in outs:
+        assert rel(rows(res[k], idx), ref[k]) <= F32C_TOL, k
+    r64 = P.forward(*dev(x, a, s, nu), outputs=outs, precision="f64")
+    for k in outs:
+        assert rel(rows(r64[k], idx), ref[k]) <= 1e-12, k
+
+
+def test_config2_leapfrog_rk4_steps(gpu, leapfrog):
+    """Three RK4 steps (dt = 0.01) of the coupled system in FP32c and in FP64
+    from the same state stay together; on the FP32c-advanced state the FP32c
+    outputs agree with the fused FP64 pass."""
+    x, a, s, nu = leapfrog
+    print("[config2] leapfrog RK4 steps f32c vs f64", flush=True)
+    pos, al, sig, nut = dev(x, a, s, nu)
+    st = {}
+    with torch.no_grad():
+        for prec in ("f32c", "f64"):
+            xx, aa = pos, al
+            for _ in range(3):
+                xx, aa = AG.rk4_coupled_step(xx, aa, sig, nut, 0.01, precision=prec, pse=True)
+            st[prec] = (xx, aa)
+    (x32, a32), (x64, a64) = st["f32c"], st["f64"]
+    assert float((x32 - x64).norm() / x64.norm()) <= 1e-9
+    assert float((a32 - a64).norm() / a64.norm()) <= 1e-7
+    outs = ("vel", "stretch", "pse")
+    r32 = P.forward(x32, a32, sig, nut, outputs=outs, precision="f32c")
+    r64 = P.forward(x32, a32, sig, nut, outputs=outs, precision="f64")
+    for k in outs:
+        assert float((r32[k] - r64[k]).norm() / r64[k].norm()) <= F32C_TOL, k
+    # the linear impulse (vpmModule.py:264-267) of both runs agrees
+    I32 = O.total_linear_impulse(P.from_soa(x32), P.from_soa(a32))
+    I64 = O.total_linear_impulse(P.from_soa(x64), P.from_soa(a64))
+    assert rel(I32, I64) <= 1e-9
+
+
+# ------------------------------------------------------------------ config 3
+@pytest.mark.parametrize("logn", [16, 18])
+def test_config3_cloud_all_passes(gpu, logn):
+    n = 1 << logn
+    x, a, s, nu = FM.random_cloud(n)
+    print(f"[config3] cloud N=2^{logn} f32c / f64 / plain vs FP64 port", flush=True)
+    d = dev(x, a, s, nu)
+    idx = strided(n, 512)
+    ref = port(x, a, s, nu, idx, ("vel", "stretch"))
+    r32 = P.forward(*d, outputs=("vel", "stretch"), precision="f32c")
+    r64 = P.forward(*d, outputs=("vel", "stretch"), precision="f64")
+    for k in ("vel", "stretch"):
+        assert rel(rows(r32[k], idx), ref[k]) <= F32C_TOL, (logn, k)
+        assert rel(rows(r64[k], idx), ref[k]) <= 1e-12, (logn, k)
+    if logn == 16:
+        u = P.selfinduced_velocity(d[0], d[1], d[2])
+        S = P.strength_gradients(d[0], d[1], d[2])
+        assert rel(rows(u, idx), ref["vel"]) <= 1e-12
+        assert rel(rows(S, idx), ref["stretch"]) <= 1e-12
+
+
+def test_config3_plain_fp64_at_full_size(gpu):
+    """The plain FP64 drop-in kernels at N = 2^20 (the bench's comparison
+    leg) against the FP64 port on 256 strided targets."""
+    n = 1 << 20
+    x, a, s, nu = FM.random_cloud(n)
+    print("[config3] cloud N=2^20 plain FP64 vs FP64 port", flush=True)
+    d = dev(x, a, s, nu)
+    u = P.selfinduced_velocity(d[0], d[1], d[2])
+    S = P.strength_gradients(d[0], d[1], d[2])
+    idx = strided(n, 256)
+    ref = port(x, a, s, nu, idx, ("vel", "stretch"))
+    assert rel(rows(u, idx), ref["vel"]) <= 1e-12
+    assert rel(rows(S, idx), ref["stretch"]) <= 1e-12
+
+
+# ------------------------------------------------------------------ config 4
+def test_config4_ring_forward_and_probes(gpu):
+    x, a, s, nu = FM.torus_ring(2048, nu=1e-3)
+    assert len(x) == 131072
+    print("[config4] ring N=131072 f32c vel+stretch+pse; probes", flush=True)
+    d = dev(x, a, s, nu)
+    outs = ("vel", "stretch", "pse")
+    res = P.forward(*d, outputs=outs, precision="f32c")
+    idx = strided(len(x), 512)
+    ref = port(x, a, s, nu, idx, outs)
+    for k in outs:
+        assert rel(rows(res[k], idx), ref[k]) <= F32C_TOL, k
+    y = FM.ring_probes(4096)[::8]  # 512 of the 4096 probes
+    u = P.induced_velocity_at(P.to_soa(y), d[0], d[1], d[2])
+    assert rel(P.from_soa(u), O.induced_velocity_at(y, x, a, s)) <= 1e-12
+
+
+# ------------------------------------------------------------------ config 5
+def _batch_vs_port(bs, sets, m, tag):
+    """Every set's velocity and stretching against the FP64 port, its PSE
+    (GPU-plugin form, nu_i = the set's nu + nu_t) against the NumPy oracle, on
+    m strided targets per set."""
+    worst = 0.0
+    for k, (_, _, sg, _) in enumerate(sets):
+        sl = bs.set_slice(k)
+        x = P.from_soa(bs.x[:, sl])
+        a = P.from_soa(bs.a[:, sl])
+        nu = bs.nu_eff[sl].cpu().numpy()
+        idx = strided(len(x), m)
+        ref = port(x, a, sg, nu, idx, ("vel", "stretch", "pse"))
+        for key, t in (("vel", bs.vel), ("stretch", bs.str_), ("pse", bs.pse_)):
+            e = rel(rows(t[:, sl], idx), ref[key])
+            worst = max(worst, e)
+            assert e <= F32C_TOL, (tag, k, len(x), key, e)
+    return worst
+
+
+def test_config5_batch_against_fp64_port(gpu):
+    sets = FM.batch_instances()
+    print(f"[config5] {len(sets)} sets, one persistent launch vs FP64 port", flush=True)
+    bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True)
+    s = bs.stream.cuda_stream
+    mask = 1 | 2 | 4
+    bs.evaluate(bs.x, bs.a, mask, True, s)
+    bs.synchronize()
+    w0 = _batch_vs_port(bs, sets, 32, "step0")
+    # graph-replayed RK4 steps with the order sorted at the first step; the
+    # evaluation below reuses that order (no re-sort), as the replays do
+    bs.step()
+    bs.capture()
+    bs.replay(5)
+    bs.synchronize()
+    bs.evaluate(bs.x, bs.a, mask, False, s)
+    bs.synchronize()
+    w1 = _batch_vs_port(bs, sets, 32, "after 6 steps")
+    print(f"[config5] worst rel-L2 step 0 {w0:.2e}, after 6 steps {w1:.2e}", flush=True)

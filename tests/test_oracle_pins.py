@@ -15,7 +15,9 @@ oracle function is pinned here by something other than its own formula:
 * energy    -- the singular Biot-Savart energy (Winckelmans-Leonard) as sigma -> 0.
 * helicity  -- equals (1/4pi) sum alpha_i . u_i with the pinned velocity.
 * enstrophy -- hand-computed single-particle and far-pair values.
-* impulse   -- ring impulse Gamma pi R^2 along the axis.
+* impulse   -- ring impulse Gamma pi R^2 along the axis; angular impulse by
+  hand-computed one- and two-particle values and the x(x.a) - a|x|^2 form.
+* core spreading -- a hand value; sigma^2 additive in dt and linear in nu.
 * LES filter -- angle thresholds 45/135 degrees; C_nu recursion and clamp.
 * time step -- the strength-history trace 1, 1, 9/11, 1.033, 0.899.
 * all pair sums -- scalar double loops at tiny N, and the independent C port.
@@ -222,6 +224,46 @@ def test_ring_linear_impulse():
     assert abs(I[1]) < 1e-12 and abs(I[2]) < 1e-12
 
 
+def test_angular_impulse_hand_values():
+    # one particle at (0,1,0) with alpha = (0,0,1):
+    # x x alpha = (1,0,0), x x (1,0,0) = (0,0,-1), A = (0,0,-1)/3
+    A = O.total_angular_impulse(np.array([[0.0, 1.0, 0.0]]), np.array([[0.0, 0.0, 1.0]]))
+    assert np.array_equal(A, np.array([0.0, 0.0, -1.0]) / 3.0)
+    # two particles: (1,0,0) with (0,1,0) gives (0,-1,0); (0,0,2) with (1,0,0)
+    # gives (-4,0,0); A = (-4,-1,0)/3
+    A2 = O.total_angular_impulse(np.array([[1.0, 0, 0], [0, 0, 2.0]]),
+                                 np.array([[0.0, 1, 0], [1.0, 0, 0]]))
+    assert np.allclose(A2, np.array([-4.0, -1.0, 0.0]) / 3.0, rtol=0, atol=1e-15)
+
+
+def test_angular_impulse_is_the_bac_cab_form():
+    # x x (x x a) = x (x.a) - a |x|^2, summed independently of any cross product
+    rng = np.random.default_rng(31)
+    x = rng.normal(0, 1, (200, 3)) + [0.3, -1.2, 0.7]
+    a = rng.normal(0, 1, (200, 3))
+    ref = sum(xi * float(xi @ ai) - ai * float(xi @ xi) for xi, ai in zip(x, a)) / 3.0
+    assert np.allclose(O.total_angular_impulse(x, a), ref, rtol=1e-13, atol=1e-12)
+    # and it is not degenerate here: every component is O(1)
+    assert np.abs(ref).min() > 1e-2
+
+
+def test_core_spreading_hand_value_and_additivity():
+    # sigma = 0.1, nu = 1e-3, dt = 0.01: sqrt(0.01 + 5.34e-5) = 0.1002666445...
+    s = O.core_spreading(np.array([0.1]), np.array([1e-3]), 0.01)
+    assert s[0] == pytest.approx(0.10026664450354367, rel=1e-15)
+    # sigma^2 grows linearly in nu t: two steps of dt equal one step of 2 dt,
+    # and the growth is linear in nu
+    r = np.array([0.05, 0.2, 1.3])
+    nu = np.array([1e-3, 4e-2, 0.0])
+    two = O.core_spreading(O.core_spreading(r, nu, 0.02), nu, 0.02)
+    one = O.core_spreading(r, nu, 0.04)
+    assert np.allclose(two, one, rtol=1e-15, atol=0)
+    d1 = O.core_spreading(r, nu, 0.1) ** 2 - r ** 2
+    d2 = O.core_spreading(r, 2 * nu, 0.1) ** 2 - r ** 2
+    assert np.allclose(d2, 2 * d1, rtol=1e-12, atol=1e-18)
+    assert d1[2] == 0.0
+
+
 # ------------------------------------------------------------------ LES filter
 def test_les_angle_threshold():
     angles = np.radians([0, 44, 46, 90, 134, 136, 180])
@@ -317,3 +359,16 @@ def test_c_port_matches_numpy_oracle():
     assert rel(CB.strength_gradients_pse(x, a, s, nu), O.strength_gradients_pse(x, a, s, nu, "cpu")) < 1e-13
     tidx = np.array([3, 500, 699])
     assert rel(CB.strength_gradients(x, a, s, tidx), O.strength_gradients(x, a, s)[tidx]) < 1e-13
+
+
+def test_row_subsets_equal_the_full_evaluation():
+    """The rows= option of the self-evaluations returns exactly those rows."""
+    x, a, s, nu = rand_set(300, 21)
+    rows = np.array([0, 7, 299, 150, 7])
+    for v in ("cuda", "cpu"):
+        assert np.array_equal(O.selfinduced_velocity(x, a, s, v, rows=rows),
+                              O.selfinduced_velocity(x, a, s, v)[rows])
+        assert np.array_equal(O.strength_gradients_pse(x, a, s, nu, v, rows=rows),
+                              O.strength_gradients_pse(x, a, s, nu, v)[rows])
+    assert np.array_equal(O.strength_gradients(x, a, s, rows=rows),
+                          O.strength_gradients(x, a, s)[rows])
