@@ -49,6 +49,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_probe(which, dev):
+    """One launch of a secondary kernel (after one untimed warm-up launch),
+    for counter runs: the ring of 131072 (velocity + stretching + PSE,
+    FP32-compensated), its adjoint, or one stage-1 launch of the 96-set batch."""
+    import torch
+
+    from openonda_amd import autograd as AG
+    from openonda_amd import flow_models as FM
+    from openonda_amd import physics as P
+
+    if which in ("ring", "adjoint"):
+        x, a, s, nu = FM.torus_ring(2048, nu=1e-3)
+        rp, ra = P.to_soa(x, dev), P.to_soa(a, dev)
+        rs, rn = torch.from_numpy(s).to(dev), torch.from_numpy(nu).to(dev)
+        if which == "ring":
+            fn = lambda: P.forward(rp, ra, rs, rn, outputs=("vel", "stretch", "pse"),
+                                   precision="f32c", sigma_mode="uniform")
+        else:
+            g = torch.Generator(device=dev).manual_seed(0)
+            cots = [torch.randn(3, len(x), dtype=torch.float64, device=dev, generator=g)
+                    for _ in range(3)]
+            fn = lambda: AG.adjoint(rp, ra, rs, rn, *cots, precision="f32c", sigma_mode="uniform")
+    else:
+        from openonda_amd import batch as B
+
+        bs = B.BatchedSystem(FM.batch_instances(), dt=0.005, precision="f32c", pse=True, les=True)
+        fn = lambda: bs.evaluate(bs.x, bs.a, bs.stage_mask, True, bs.stream.cuda_stream)
+    fn()
+    torch.cuda.synchronize()
+    fn()
+    torch.cuda.synchronize()
+    log(f"[bench] pmc probe done: {which}, 2 launches")
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,14 +91,18 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--tile-config", type=int, default=-1)
-    ap.add_argument("--cpu-targets", type=int, default=16384)
+    ap.add_argument("--cpu-targets", type=int, default=32768)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="one forward step and one 256 MiB copy, for rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-probe", nargs="?", const="cloud", default=None,
+                    choices=["cloud", "ring", "adjoint", "batch"],
+                    help="for rocprofv3 --pmc runs: one launch of the kernel named (cloud: "
+                         "the headline pass and one 256 MiB copy; ring: velocity + "
+                         "stretching + PSE on the ring of 131072; adjoint: its adjoint; "
+                         "batch: one stage-1 launch over the 96 sets)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -81,6 +120,8 @@ def main(argv=None):
 
     P.require_gpu()
     dev = torch.device("cuda", 0)
+    if args.pmc_probe in ("ring", "adjoint", "batch"):
+        return pmc_probe(args.pmc_probe, dev)
     n = args.n
     log(f"[bench] building random cloud N={n}")
     x, a, s, nu = FM.random_cloud(n)
@@ -148,6 +189,9 @@ def main(argv=None):
                            else P.tile_config_default(outputs)]
     kernel_name = "fused_kernel<float, 3u, true, {}, {}, {}, {}, {}>".format(*cfg)
     traffic = profiling.load_traffic(traffic_workload(n), kernel_name)
+    # executed FP32 flops per pair of this kernel instance, from the VALU
+    # instruction counters summarised under profiles/ (FMA 2, others 1)
+    exec_fpp = profiling.load_record(traffic_workload(n), kernel_name, "exec_fp32_flop_per_pair")
 
     result = {
         "metric": METRIC,
@@ -181,6 +225,13 @@ def main(argv=None):
             "flops_per_pair": FLOPS_PER_PAIR_VEL_STRETCH,
         },
     }
+    if exec_fpp is not None:
+        ex = exec_fpp * pairs / (kern_ms * 1e-3) / 1e12
+        result["roofline"]["executed"] = {
+            "flops_per_pair": exec_fpp, "achieved": ex, "frac": ex / FP32_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "source": "SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F32 per pair of this kernel instance "
+                      "(rocprofv3 --pmc, profiles/), times this run's pairs / kernel time"}
 
     # ---- plain FP64 drop-in kernels on the same input ---------------------
     if not args.skip_plain:
@@ -263,8 +314,9 @@ def main(argv=None):
             "value": pps, "unit": "pair-interactions/s", "cores": CB.threads(), "kind": "port",
             "sample": f"velocity + stretching (two passes, as the reference runs them) for "
                       f"{len(tidx)} strided targets x {n} sources in {secs:.1f} s; "
-                      f"C/OpenMP FP64 port of ParticlesPhysics.py, -O3 -ffast-math",
-            "cpu_model": CB.host_cpu_model(),
+                      f"C/OpenMP FP64 port of ParticlesPhysics.py, -O3 -ffast-math "
+                      f"-march={CB.isa()}",
+            "cpu_model": CB.host_cpu_model(), "isa": CB.isa(),
         }
         log(f"[bench] CPU {pps:.3e} pairs/s ({secs:.1f} s); rel-L2 {rel}")
         m1 = max(1, m // 64)
@@ -313,9 +365,18 @@ def main(argv=None):
     if not args.skip_batch:
         from openonda_amd import batch as B
 
+        from openonda_amd import autotune as AT
+
         sets = FM.batch_instances()
-        bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True)
-        log(f"[bench] batch: {bs.nsets} sets, N_total={bs.n}, {bs.n_items} work items")
+        stage = L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE | L.OUT_VORT | L.OUT_ENSTROPHY
+        if AT.best_batch_config([len(q[0]) for q in sets], stage) is None:
+            log("[bench] batch: tuning the persistent launch (tile shape x grid; untimed)")
+            AT.tune_batch(sets, reps=1)
+        bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True,
+                             tile_config="auto")
+        log(f"[bench] batch: {bs.nsets} sets, N_total={bs.n}, {bs.n_items} work items, "
+            f"tile config {bs.tile_config}, grid {bs.grid} "
+            f"({'autotuned' if bs.autotuned else 'default'})")
         bs.step()          # sorts, eager
         bs.capture()
         bs.replay(1)       # warm
@@ -326,8 +387,9 @@ def main(argv=None):
         e1.record(bs.stream)
         bs.synchronize()
         bms = e0.elapsed_time(e1)
-        # per-set accuracy of one FP32-compensated evaluation against the plain FP64 kernels
-        bs.evaluate(bs.x, bs.a, 3, True, bs.stream.cuda_stream)
+        # per-set accuracy of one FP32-compensated evaluation against the plain
+        # FP64 kernels, in the sorted order the replayed steps used (no re-sort)
+        bs.evaluate(bs.x, bs.a, 3, False, bs.stream.cuda_stream)
         bs.synchronize()
         errs, per_set = [], []
         for k in range(bs.nsets):
@@ -345,6 +407,8 @@ def main(argv=None):
             "workload": f"{bs.nsets} ring/cloud sets, N_i log-uniform in [2048, 131072], "
                         f"N_total={bs.n}; RK4 (velocity+stretching+PSE, LES filter per set), "
                         f"one persistent launch per stage, HIP-graph replay",
+            "tile_config": bs.tile_config, "grid": bs.grid,
+            "launch": "autotuned" if bs.autotuned else "default",
             "steps": args.batch_steps, "ms_per_step": bms / args.batch_steps,
             "pairs_per_s": pairs_step * args.batch_steps / (bms * 1e-3),
             "rel_l2_max_vs_fp64": max(errs), "rel_l2_median_vs_fp64": float(np.median(errs)),

@@ -40,6 +40,7 @@ extern "C" {
 #define VPM_ERR_LAUNCH (-3)     /* HIP runtime or launch failure */
 #define VPM_ERR_WORKSPACE (-4)  /* caller workspace too small */
 #define VPM_ERR_UNSUPPORTED (-5)
+#define VPM_ERR_NONFINITE (-6)  /* input holds NaN/Inf, or a radius <= 0 (vpm_scan_particles) */
 
 /* Which reference plugin's arithmetic to reproduce.
  * CUDA: OpenONDA/solvers/VPM/ParticlesPhysicsCuda.py (the GPU plugin).
@@ -185,10 +186,12 @@ size_t vpm_forward_workspace_bytes(int64_t n, const vpm_forward_opts_t* opts);
  * repeated on one prepared workspace while the particles are unchanged. */
 int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
                 vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream);
+
 int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
                         void* workspace, size_t ws_bytes, void* stream);
 int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
                     vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream);
+
 
 /* Tile configurations the fused F32C kernel is instantiated for (used by the
  * autotuner).  Writes block threads, targets per lane, source tile, sub-tile
@@ -306,6 +309,19 @@ int vpm_core_spreading(int64_t n, double* sigma, const double* nu, const double*
 /* alpha <- 0.95 alpha + 0.05 (omega/|omega|)|alpha|, vpmModule.py:867-883. */
 int vpm_relax_strengths(int64_t n, int64_t ld, double* strengths, const double* vorticity,
                         void* stream);
+/* Particle-set scan for inputs the pair sums cannot take: ORs VPM_BAD_*
+ * bits into *flags (device memory; the caller zeroes it and reads it after
+ * the stream reaches this point -- a nonzero value maps to
+ * VPM_ERR_NONFINITE).  No reference counterpart: the reference plugins
+ * propagate NaN and Inf silently (e.g. the energy's division by |r| at
+ * coincident particles, ParticlesPhysicsCuda.py:595-600, and the LES
+ * filter's division by |mean vorticity|, vpmModule.py:936-941). */
+#define VPM_BAD_POSITION 1u  /* a position is NaN/Inf */
+#define VPM_BAD_STRENGTH 2u  /* a strength is NaN/Inf */
+#define VPM_BAD_RADIUS 4u    /* a radius is NaN/Inf or <= 0 */
+#define VPM_BAD_VISCOSITY 8u /* a viscosity (if given) is NaN/Inf or < 0 */
+int vpm_scan_particles(const vpm_particles_t* p, uint32_t* flags, void* stream);
+
 /* Workspace of vpm_diagnostics / vpm_les_update. */
 size_t vpm_state_workspace_bytes(int64_t n);
 /* out10 = [linear impulse (3), angular impulse (3), sum alpha (3), sum |alpha|]

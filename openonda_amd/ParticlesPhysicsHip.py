@@ -17,6 +17,10 @@ Differences from the reference, all deliberate:
  * Inputs that are not float64 raise ``TypeError`` and bad shapes raise
    ``ValueError`` (the reference's explicit Numba signatures reject the
    former, ``ParticlesPhysicsCuda.py:39``).
+ * NaN or Inf in any input, a radius <= 0 or a viscosity < 0 raises
+   ``NonFiniteError`` (a ``ValueError``, status ``VPM_ERR_NONFINITE``) before
+   any GPU work; the reference returns NaN.  ``CHECK_FINITE = False`` skips
+   the check.
 
 ``VARIANT`` selects whose arithmetic is reproduced: "cuda" (this module's
 default, the GPU plugin) or "cpu" (``ParticlesPhysics.py``, the Numba CPU
@@ -37,13 +41,20 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from . import _lib as L
 from . import physics as P
 
 VARIANT = "cuda"
 PRECISION = "plain"
+# Inputs are checked on the host before the upload: NaN/Inf in any array, a
+# radius <= 0 or a viscosity < 0 raises NonFiniteError (status
+# VPM_ERR_NONFINITE) instead of returning NaN as the reference does.
+CHECK_FINITE = True
 
 
-def _arr(a, name, cols=3):
+def _arr(a, name, cols=3, bad=0):
+    """Validates one host array; ``bad`` is its VPM_BAD_* bit for the
+    finiteness check (0: none)."""
     if not isinstance(a, np.ndarray):
         a = np.asarray(a)
     if a.dtype != np.float64:
@@ -53,6 +64,14 @@ def _arr(a, name, cols=3):
             raise ValueError(f"{name} must have shape (N,), got {a.shape}")
     elif a.ndim != 2 or a.shape[1] != cols:
         raise ValueError(f"{name} must have shape (N, {cols}), got {a.shape}")
+    if CHECK_FINITE and bad:
+        ok = np.isfinite(a).all()
+        if ok and bad == L.BAD_RADIUS:
+            ok = bool((a > 0).all())
+        elif ok and bad == L.BAD_VISCOSITY:
+            ok = bool((a >= 0).all())
+        if not ok:
+            raise L.NonFiniteError(name, bad)
     return a
 
 
@@ -65,19 +84,26 @@ def _soa(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
 
 
-def _src(x, a, s, dev, nu=None):
-    x = _arr(x, "particles_positions")
-    a = _arr(a, "particles_strengths")
-    s = _arr(s, "particles_radii", None)
+def _src(x, a, s, nu=None, targets=None):
+    """Validates every array on the host (before any GPU call), then uploads:
+    returns ([targets,] x, a, s[, nu]) as device tensors."""
+    y = _arr(targets, "target_positions", bad=L.BAD_POSITION) if targets is not None else None
+    x = _arr(x, "particles_positions", bad=L.BAD_POSITION)
+    a = _arr(a, "particles_strengths", bad=L.BAD_STRENGTH)
+    s = _arr(s, "particles_radii", None, bad=L.BAD_RADIUS)
     n = x.shape[0]
     if a.shape[0] != n or s.shape[0] != n:
         raise ValueError("positions, strengths and radii must have the same length")
-    out = [_soa(x, dev), _soa(a, dev), torch.from_numpy(np.ascontiguousarray(s)).to(dev)]
     if nu is not None:
-        nu = _arr(nu, "viscosities", None)
+        nu = _arr(nu, "viscosities", None, bad=L.BAD_VISCOSITY)
         if nu.shape[0] != n:
             raise ValueError("viscosities must have the same length as positions")
+    dev = _dev()
+    out = [_soa(x, dev), _soa(a, dev), torch.from_numpy(np.ascontiguousarray(s)).to(dev)]
+    if nu is not None:
         out.append(torch.from_numpy(np.ascontiguousarray(nu)).to(dev))
+    if y is not None:
+        out.insert(0, _soa(y, dev))
     return out
 
 
@@ -100,16 +126,14 @@ def _fused():
 def get_induced_velocity_at(target_positions, particles_positions, particles_strengths,
                             particles_radii):
     """Velocity at targets, (M,3).  ParticlesPhysicsCuda.py:11-34."""
-    dev = _dev()
-    y = _soa(_arr(target_positions, "target_positions"), dev)
-    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    y, x, a, s = _src(particles_positions, particles_strengths, particles_radii,
+                      targets=target_positions)
     return _host(P.induced_velocity_at(y, x, a, s))
 
 
 def get_selfinduced_velocity(particles_positions, particles_strengths, particles_radii):
     """Self-induced velocity, (N,3).  ParticlesPhysicsCuda.py:103-127."""
-    dev = _dev()
-    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii)
     prec = _fused()
     if prec:
         return _host(P.forward(x, a, s, outputs=("vel",), precision=prec)["vel"])
@@ -119,16 +143,14 @@ def get_selfinduced_velocity(particles_positions, particles_strengths, particles
 def get_vorticity_field_at(target_positions, particle_positions, particle_strengths,
                            particle_radii):
     """Smoothed vorticity at targets, (M,3).  ParticlesPhysicsCuda.py:198-219."""
-    dev = _dev()
-    y = _soa(_arr(target_positions, "target_positions"), dev)
-    x, a, s = _src(particle_positions, particle_strengths, particle_radii, dev)
+    y, x, a, s = _src(particle_positions, particle_strengths, particle_radii,
+                      targets=target_positions)
     return _host(P.vorticity_field_at(y, x, a, s, VARIANT))
 
 
 def get_strength_gradients(particle_positions, particle_strengths, particle_radii):
     """Stretching dalpha/dt, (N,3).  ParticlesPhysicsCuda.py:351-370."""
-    dev = _dev()
-    x, a, s = _src(particle_positions, particle_strengths, particle_radii, dev)
+    x, a, s = _src(particle_positions, particle_strengths, particle_radii)
     prec = _fused()
     if prec:
         return _host(P.forward(x, a, s, outputs=("stretch",), precision=prec)["stretch"])
@@ -137,8 +159,7 @@ def get_strength_gradients(particle_positions, particle_strengths, particle_radi
 
 def get_strength_gradients_PSE(positions, strengths, radii, viscosities):
     """PSE dalpha/dt, (N,3).  ParticlesPhysicsCuda.py:452-473."""
-    dev = _dev()
-    x, a, s, nu = _src(positions, strengths, radii, dev, viscosities)
+    x, a, s, nu = _src(positions, strengths, radii, viscosities)
     prec = _fused()
     if prec:
         return _host(P.forward(x, a, s, nu, outputs=("pse",), precision=prec)["pse"])
@@ -147,22 +168,19 @@ def get_strength_gradients_PSE(positions, strengths, radii, viscosities):
 
 def get_total_kinetic_energy(particles_positions, particles_strengths, particles_radii):
     """Kinetic energy (float).  ParticlesPhysicsCuda.py:537-558."""
-    dev = _dev()
-    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii)
     return float(P.total_kinetic_energy(x, a, s, VARIANT).item())
 
 
 def get_total_helicity(particles_positions, particles_strengths, particles_radii):
     """Helicity (float).  ParticlesPhysicsCuda.py:647-668."""
-    dev = _dev()
-    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii)
     return float(P.total_helicity(x, a, s, VARIANT).item())
 
 
 def get_total_enstrophy(particles_positions, particles_strengths, particles_radii):
     """Enstrophy (float).  ParticlesPhysicsCuda.py:756-777."""
-    dev = _dev()
-    x, a, s = _src(particles_positions, particles_strengths, particles_radii, dev)
+    x, a, s = _src(particles_positions, particles_strengths, particles_radii)
     prec = _fused()
     if prec:
         return float(P.forward(x, a, s, outputs=("enstrophy",), precision=prec)["enstrophy"].item())

@@ -18,6 +18,15 @@ VPM_ERR_DEVICE = -2
 VPM_ERR_LAUNCH = -3
 VPM_ERR_WORKSPACE = -4
 VPM_ERR_UNSUPPORTED = -5
+VPM_ERR_NONFINITE = -6
+
+# vpm_scan_particles flag bits
+BAD_POSITION = 1
+BAD_STRENGTH = 2
+BAD_RADIUS = 4
+BAD_VISCOSITY = 8
+BAD_NAMES = {BAD_POSITION: "positions", BAD_STRENGTH: "strengths", BAD_RADIUS: "radii",
+             BAD_VISCOSITY: "viscosities"}
 
 VARIANT_CUDA = 0
 VARIANT_CPU = 1
@@ -45,6 +54,19 @@ class VpmError(RuntimeError):
     def __init__(self, func, code, msg):
         super().__init__(f"{func} failed with status {code}: {msg}")
         self.code = code
+
+
+class NonFiniteError(ValueError):
+    """Particle data the pair sums cannot take: NaN/Inf positions, strengths,
+    radii or viscosities, radii <= 0 or viscosities < 0.  ``code`` is
+    VPM_ERR_NONFINITE, ``bits`` the VPM_BAD_* flags found."""
+
+    def __init__(self, where, bits):
+        names = [v for k, v in BAD_NAMES.items() if bits & k]
+        super().__init__(f"{where}: non-finite or out-of-range {', '.join(names)} "
+                         f"(status {VPM_ERR_NONFINITE})")
+        self.code = VPM_ERR_NONFINITE
+        self.bits = bits
 
 
 class Particles(C.Structure):
@@ -119,6 +141,7 @@ SIGNATURES = {
     "vpm_relax_strengths": (C.c_int, [_I64, _I64, _VP, _VP, _VP]),
     "vpm_state_workspace_bytes": (_SZ, [_I64]),
     "vpm_diagnostics": (C.c_int, [_I64, _I64, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "vpm_scan_particles": (C.c_int, [_VP, _VP, _VP]),
     "vpm_les_update": (C.c_int, [_I64, _I64, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP, _SZ, _VP]),
     "vpm_graph_begin": (C.c_int, [_VP]),
     "vpm_graph_end": (C.c_int, [_VP, _P(C.c_void_p)]),
@@ -142,8 +165,14 @@ def lib_path() -> Path:
 
 
 def load() -> C.CDLL:
-    """Loads libvpm_hip.so once; raises if it is absent or incomplete."""
+    """Loads libvpm_hip.so once; raises if it is absent or incomplete.
+
+    torch is imported first: the PyTorch-ROCm wheel carries its own HIP
+    runtime under the same soname, and the library must bind to that one
+    (a second runtime in the process would not share streams or memory)."""
     global _lib
+    import torch  # noqa: F401  (binds libamdhip64.so.7 to torch's runtime)
+
     with _lock:
         if _lib is not None:
             return _lib

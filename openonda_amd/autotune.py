@@ -8,6 +8,10 @@ the same input, and keeps the fastest one within the accuracy bound.  Results
 are cached in a JSON file keyed by (kernel, precision, output set, radius
 mode, N rounded to a power of two); :func:`best_config` reads the cache and
 ``physics.forward(..., tile_config="auto")`` uses it.
+
+:func:`tune_batch` does the same for the batched persistent launch over
+(tile shape x persistent grid size), keyed by output mask and total pair
+count; ``BatchedSystem(tile_config="auto")`` uses its choice.
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ import threading
 import warnings
 from pathlib import Path
 
+import numpy as np
 import torch
 
 from . import physics as P
@@ -144,6 +149,94 @@ def device_tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mo
     L.call("vpm_autotune", C.byref(p), C.byref(opts), C.byref(fo), ws.data_ptr(), ws.numel(),
            P._stream(pos), int(reps), C.byref(best), ms)
     return best.value, [float(v) for v in ms]
+
+
+# ---------------------------------------------------------------- batch
+def _batch_key(sizes, mask, precision="f32c"):
+    pairs = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
+    bucket = int(round(math.log2(pairs))) if pairs > 1 else 0
+    return f"batch|{precision}|mask{int(mask)}|pairs2^{bucket}|sets{len(sizes)}"
+
+
+def best_batch_config(sizes, mask, precision="f32c", path=None):
+    """Cached (tile_config, grid) of the batched launch for sets of these
+    sizes and this output mask (total pair count rounded to a power of two),
+    or None."""
+    with _lock:
+        c = load_cache(path)
+    e = c.get(_batch_key(sizes, mask, precision))
+    return (int(e["config"]), int(e["grid"])) if e else None
+
+
+def tune_batch(sets, pse=True, les=True, configs=None, grids=None, reps=2, max_rel=1e-6,
+               path=None, save=True):
+    """Times the batched persistent launch (one stage-1 evaluation: velocity,
+    stretching and, as requested, PSE and the LES outputs, over every set) for
+    each tile configuration x persistent grid size on these sets, checks every
+    set's outputs against the FP64 batched pass (rel-L2 <= ``max_rel`` per
+    set and output), and caches the fastest accurate (config, grid) under the
+    sets' output mask and total pair count.  ``grids``: persistent workgroup
+    counts (0: as many as fit; default 0 and 1-3 per CU).  Returns (best,
+    report); best is None (nothing cached, a warning) when no candidate meets
+    the bound."""
+    from . import batch as B
+    from . import _lib as L
+
+    P.require_gpu()
+    ref = B.BatchedSystem(sets, dt=0.0, precision="f64", pse=pse, les=les)
+    mask = ref.stage_mask
+    s = ref.stream.cuda_stream
+    ref.evaluate(ref.x, ref.a, mask, True, s)
+    ref.synchronize()
+    outs = [("vel", ref.vel), ("stretch", ref.str_)] + ([("pse", ref.pse_)] if pse else []) + \
+        ([("vort", ref.vort)] if les else [])
+    refv = {k: v.clone() for k, v in outs}
+    refe = ref.enstrophy.clone() if les else None
+    del ref
+    bs = B.BatchedSystem(sets, dt=0.0, precision="f32c", pse=pse, les=les)
+    cus = torch.cuda.get_device_properties(bs.device).multi_processor_count
+    configs = list(range(len(P.tile_configs()))) if configs is None else list(configs)
+    grids = [0, cus, 2 * cus, 3 * cus] if grids is None else list(grids)
+    sl = [bs.set_slice(k) for k in range(bs.nsets)]
+    report = []
+    for cfg in configs:
+        for grid in grids:
+            bs.set_launch(cfg, grid)
+            run = lambda: bs.evaluate(bs.x, bs.a, mask, False, s)
+            bs.evaluate(bs.x, bs.a, mask, True, s)  # sorts (first time) and warms
+            bs.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(bs.stream)
+            for _ in range(reps):
+                run()
+            e1.record(bs.stream)
+            bs.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            worst = 0.0
+            for k, v in refv.items():
+                got = dict(vel=bs.vel, stretch=bs.str_, pse=bs.pse_, vort=bs.vort)[k]
+                for q in sl:
+                    nr = float(v[:, q].norm())
+                    if nr > 0:
+                        worst = max(worst, float((got[:, q] - v[:, q]).norm()) / nr)
+            if refe is not None:
+                worst = max(worst, float(((bs.enstrophy - refe).abs() /
+                                          refe.abs().clamp_min(1e-300)).max()))
+            report.append({"config": cfg, "grid": grid, "ms": ms, "rel_l2": worst})
+    ok = [r for r in report if r["rel_l2"] <= max_rel]
+    if not ok:
+        warnings.warn(f"tune_batch: no launch reached rel-L2 <= {max_rel:g}; nothing cached",
+                      RuntimeWarning, stacklevel=2)
+        return None, report
+    best = min(ok, key=lambda r: r["ms"])
+    if save:
+        with _lock:
+            c = load_cache(path)
+            c[_batch_key(bs.sizes, mask)] = {
+                "config": best["config"], "grid": best["grid"], "ms": best["ms"],
+                "rel_l2": best["rel_l2"], "sets": int(bs.nsets), "candidates": report}
+            save_cache(c, path)
+    return best, report
 
 
 def main(argv=None):

@@ -48,7 +48,13 @@ class BatchedSystem:
     """Particle sets [(x (N_i,3), alpha (N_i,3), sigma (N_i,), nu (N_i,)), ...]."""
 
     def __init__(self, sets, dt, precision="f32c", tile_config=-1, pse=True, les=True,
-                 device=None):
+                 device=None, grid=0, resort_every=0, autotune_cache=None):
+        """``tile_config``: -1 (the library's default shape for the batch),
+        an index of physics.tile_configs(), or "auto" (the batch autotuner's
+        cached (shape, grid) for this output set and total pair count, else
+        the default).  ``grid``: persistent workgroups (0: as many as fit on
+        the device).  ``resort_every``: re-sort the sets by Morton key every
+        that many steps, eager or replayed (0: only at the first step)."""
         P.require_gpu()
         if precision not in ("f32c", "f64"):
             raise ValueError("precision must be 'f32c' or 'f64'")
@@ -57,13 +63,20 @@ class BatchedSystem:
         self.precision = precision
         self.pse = pse
         self.les = les
+        self.resort_every = int(resort_every)
         # one tile shape for every stage (the work items count its target
         # blocks), chosen for the heaviest stage's outputs
-        stage = (L.OUT_VEL | L.OUT_STRETCH | (L.OUT_PSE if pse else 0)
-                 | (L.OUT_VORT | L.OUT_ENSTROPHY if les else 0))
-        self.tile_config = (int(tile_config) if tile_config >= 0
-                            else int(L.load().vpm_batch_tile_config_default(stage)))
+        self.stage_mask = (L.OUT_VEL | L.OUT_STRETCH | (L.OUT_PSE if pse else 0)
+                           | (L.OUT_VORT | L.OUT_ENSTROPHY if les else 0))
         sizes = [len(s[0]) for s in sets]
+        self.autotuned = None
+        if tile_config == "auto":
+            from . import autotune
+
+            self.autotuned = autotune.best_batch_config(sizes, self.stage_mask, path=autotune_cache)
+            tile_config, grid = self.autotuned if self.autotuned else (-1, grid)
+        self.tile_config = (int(tile_config) if tile_config >= 0
+                            else int(L.load().vpm_batch_tile_config_default(self.stage_mask)))
         for k, s in enumerate(sets):
             sig = np.asarray(s[2])
             if len(sig) and not np.all(sig == sig[0]):
@@ -85,24 +98,8 @@ class BatchedSystem:
         off = np.zeros(self.nsets + 1, dtype=np.int32)
         off[1:] = np.cumsum(self.sizes)
         self.offsets = off
-        if precision == "f64":
-            tpi = 256
-        else:
-            cfgs = P.tile_configs()
-            b, r = cfgs[self.tile_config][:2]
-            tpi = b * r
-        item_set, item_block, first, count = work_list(self.sizes, tpi)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(d)
-        self._desc_t = [t(off), t(np.repeat(np.arange(self.nsets), self.sizes)), t(item_set),
-                        t(item_block), t(first), t(count)]
-        self.n_items = len(item_set)
-        self.desc = L.BatchDesc(self.nsets, self.n_items, 0, *[x.data_ptr() for x in self._desc_t])
-        lib = L.load()
-        wsb = lib.vpm_batch_workspace_bytes(n, self.nsets, self.n_items)
-        if wsb == 0:
-            raise L.VpmError("vpm_batch_workspace_bytes", -3, lib.vpm_last_error().decode())
-        self.ws = torch.empty(wsb, dtype=torch.uint8, device=d)
-        self.wsb = wsb
+        self.ws, self.wsb = None, 0
+        self.set_launch(self.tile_config, grid)
         e = lambda: torch.zeros((3, n), dtype=torch.float64, device=d)
         self.kx = [e() for _ in range(4)]
         self.ka = [e() for _ in range(4)]
@@ -116,6 +113,47 @@ class BatchedSystem:
         self.graph = None
         self.steps_done = 0
         self._sorted = False
+
+    def set_launch(self, tile_config, grid=0):
+        """Selects the persistent launch's tile shape and grid: rebuilds the
+        work list (its items are the shape's target blocks) and grows the
+        workspace if needed; the sorted order in the workspace is kept.  A
+        captured graph is dropped (it holds the old launch)."""
+        d = self.device
+        self.tile_config = int(tile_config)
+        self.grid = int(grid)
+        if self.precision == "f64":
+            tpi = 256
+        else:
+            b, r = P.tile_configs()[self.tile_config][:2]
+            tpi = b * r
+        item_set, item_block, first, count = work_list(self.sizes, tpi)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(d)
+        self._desc_t = [t(self.offsets), t(np.repeat(np.arange(self.nsets), self.sizes)),
+                        t(item_set), t(item_block), t(first), t(count)]
+        self.n_items = len(item_set)
+        self.desc = L.BatchDesc(self.nsets, self.n_items, self.grid,
+                                *[x.data_ptr() for x in self._desc_t])
+        lib = L.load()
+        wsb = lib.vpm_batch_workspace_bytes(self.n, self.nsets, self.n_items)
+        if wsb == 0:
+            raise L.VpmError("vpm_batch_workspace_bytes", -3, lib.vpm_last_error().decode())
+        if self.ws is None or wsb > self.ws.numel():
+            old = self.ws
+            if old is not None:
+                # the system's stream may still be using the old workspace
+                self.stream.synchronize()
+            self.ws = torch.empty(wsb, dtype=torch.uint8, device=d)
+            if old is not None:  # keep the sorted order (perm, records)
+                self.ws[:old.numel()].copy_(old)
+                # the copy runs on the current stream, the launches on the
+                # system's: finish it before any launch reads the workspace
+                # (the sort's scratch moves when the layout grows)
+                torch.cuda.current_stream(d).synchronize()
+        self.wsb = self.ws.numel()
+        if getattr(self, "graph", None) is not None:
+            L.call("vpm_graph_destroy", self.graph)
+            self.graph = None
 
     # ------------------------------------------------------------ primitives
     def _parts(self, x, a):
@@ -131,8 +169,14 @@ class BatchedSystem:
         return m
 
     def evaluate(self, x, a, mask, resort, stream, vel=None, stretch=None):
-        """prepare + one persistent launch.  Outputs go to ``vel``/``stretch``
+        """prepare + one persistent launch on ``stream`` (the system's),
+        after the work already issued on the current stream (inputs written
+        there, outputs allocated there).  Outputs go to ``vel``/``stretch``
         (default self.vel/self.str_), self.pse_, self.vort, self.enstrophy."""
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        self._evaluate(x, a, mask, resort, stream, vel, stretch)
+
+    def _evaluate(self, x, a, mask, resort, stream, vel=None, stretch=None):
         p = self._parts(x, a)
         L.call("vpm_batch_prepare", C.byref(p), C.byref(self.desc), 1 if resort else 0,
                self.ws.data_ptr(), self.wsb, stream)
@@ -150,7 +194,7 @@ class BatchedSystem:
 
     def _stage(self, k, x, a, resort, stream):
         # kx = u, ka = S (+ D)
-        self.evaluate(x, a, self._mask(k == 0), resort, stream, self.kx[k], self.ka[k])
+        self._evaluate(x, a, self._mask(k == 0), resort, stream, self.kx[k], self.ka[k])
         if self.pse:
             self._axpy(self.ka[k], self.pse_, 1.0, self.ka[k], stream)
         if k == 0 and self.les:
@@ -171,9 +215,14 @@ class BatchedSystem:
                    ks[1].data_ptr(), ks[2].data_ptr(), ks[3].data_ptr(), dt, y.data_ptr(), stream)
 
     # ------------------------------------------------------------ public
+    def _resort_due(self):
+        return self.resort_every > 0 and self.steps_done % self.resort_every == 0
+
     def step(self, resort=None):
-        """One eager step on the system's stream (sorts on the first call)."""
-        resort = (not self._sorted) if resort is None else resort
+        """One eager step on the system's stream (sorts on the first call and
+        every ``resort_every`` steps)."""
+        if resort is None:
+            resort = not self._sorted or self._resort_due()
         torch.cuda.current_stream(self.device).synchronize()
         self._step(resort, self.stream.cuda_stream)
         self._sorted = True
@@ -198,10 +247,17 @@ class BatchedSystem:
         self.graph = g.value
 
     def replay(self, steps=1):
+        """Replays the captured step; where a re-sort is due
+        (``resort_every``) that step runs eagerly with the sort instead (the
+        graph reads the sorted order from the workspace, so it stays valid)."""
         if self.graph is None:
             self.capture()
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
         for _ in range(steps):
-            L.call("vpm_graph_launch", self.graph, self.stream.cuda_stream)
+            if self._resort_due():
+                self._step(True, self.stream.cuda_stream)
+            else:
+                L.call("vpm_graph_launch", self.graph, self.stream.cuda_stream)
             self.steps_done += 1
 
     def synchronize(self):

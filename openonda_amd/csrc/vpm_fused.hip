@@ -31,6 +31,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdint>
 #include <type_traits>
 
 #include "vpm_common.h"
@@ -61,6 +62,12 @@ struct FusedArgs {
   int g64_mode;         // VPM_G64_*: per-pair FP64 alpha_j sum of the stretching
   int64_t pld;
   int chunk;
+  // Far-field records of the FP32 pass (far_records_kernel): per chunk of
+  // SUB sorted sources its centre and scaled half-extent (far_box, 6 doubles)
+  // and per source FP32 chunk-centred offsets, strength, moment and (PSE)
+  // low parts (far_rec, FarRec<OUT>::kStride floats).  Null where unused.
+  const float* far_rec;
+  const double* far_box;
 };
 
 template <typename Real>
@@ -116,7 +123,9 @@ struct Tgt {
 
 // Accumulator slots; kH is the moment sum of the far-field form (folded into
 // kU's FP64 totals per sub-tile, never stored).
-enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kW = 12, kE = 15, kH = 18, kNAcc = 21 };
+// kT9 is the far-field form's sum of the PSE weights (folded into kP's
+// partials per sub-tile, never stored).
+enum : int { kU = 0, kG = 3, kSd = 6, kP = 9, kW = 12, kE = 15, kH = 18, kT9 = 21, kNAcc = 22 };
 
 template <unsigned OUT>
 struct Slots {
@@ -371,11 +380,119 @@ __device__ __forceinline__ Real self_g(Real hs, const PairConst<Real>& k) {
 // emulation: 1.27e-7 / 1.02e-6 either way, 88% of pairs far).
 // p = a_i x x_i per target.
 constexpr double kFarGap = 4.0;
+// The far records are centred on their own chunk, so |x_j| is at most the
+// chunk's radius and x_i x G + H is no cancellation at any gap; the gap is
+// still 4 radii: the far form's FP32 triple product is what limits it.  At
+// 2 radii the ring of 131072 with PSE ran 17.6 -> 16.2 ms, but the batch's
+// thin rings (126,592 particles, |S| ~ 2e-3 |alpha||u|) lost the 1e-6 bar
+// in the stretching (1.3e-6), as did the single thin ring of 126,592.
+constexpr double kFarGapS = 4.0;
 
 template <typename Real>
 struct Mom {  // a source's moment a_j x x_j
   Real x, y, z;
 };
+
+// Far-field record of one source in the FP32 pass, relative to the centre of
+// its chunk of SUB sorted sources and scaled by 1/sigma: offset (3),
+// strength (3), moment a_j x x_j (3) and, with PSE, the strength's low parts
+// a_j - fp32(a_j) (3).  A wave reads the records of a far chunk through
+// scalar loads (every lane pairs with the same source), so they reach the
+// pair instructions as scalar operands with no LDS staging, no LDS reads and
+// no barrier; the target's offset to the chunk centre, and a_i x x_i, are
+// formed once per chunk.  Relative to the chunk centre |x_j| is at most the
+// chunk's radius, so x_i x G + H is not a cancellation on a far chunk.
+constexpr int kMinFarChunk = 64, kMaxFarChunk = 128;  // SUB of the shapes that use them
+#ifndef VPM_FAR_GROUP
+#define VPM_FAR_GROUP 4
+#endif
+constexpr int kFarGroup = VPM_FAR_GROUP;  // far records per scalar-load group
+template <unsigned OUT>
+struct FarRec {
+  static constexpr int kStride = (OUT & VPM_OUT_PSE) ? 12 : 9;
+};
+
+// Constant-address-space view of the records: a load through it with a
+// wave-uniform address becomes a scalar (s_load) load.
+typedef __attribute__((address_space(4))) const float far_float_t;
+
+// Builds the far records of chunk c (sources [c*CH, c*CH + CH)) of a set of
+// n sorted records: one workgroup of CH lanes.  Uniform radius sigma0 (scale
+// = 1/sigma0, as the pass computes it).  Lanes past n write zero-strength
+// padding far away, which contributes exactly 0.  rec / box point at the
+// chunk's own records and box.
+template <int CH>
+__device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, int n, int c,
+                                                  int stride, double sigma0,
+                                                  float* __restrict__ rec,
+                                                  double* __restrict__ box) {
+  static_assert(CH % 64 == 0 && CH <= 256, "chunk");
+  __shared__ double red[6][CH / 64];
+  __shared__ double cen[3];
+  const int k = c * CH + static_cast<int>(threadIdx.x);
+  const bool ok = k < n;
+  const double scale = 1.0 / sigma0;
+  Src8 g{};
+  if (ok) g = src[k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  {
+    double v[3] = {g.x, g.y, g.z};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double lo = wave_min(ok ? v[q] : INFINITY), hi = wave_max(ok ? v[q] : -INFINITY);
+      if (lane == 0) {
+        red[q][wave] = lo;
+        red[3 + q][wave] = hi;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    double lo = red[q][0], hi = red[3 + q][0];
+    for (int w = 1; w < CH / 64; ++w) {
+      lo = fmin(lo, red[q][w]);
+      hi = fmax(hi, red[3 + q][w]);
+    }
+    cen[q] = 0.5 * (lo + hi);
+    box[q] = cen[q];
+    box[3 + q] = 0.5 * (hi - lo) * scale;
+  }
+  __syncthreads();
+  float* r = rec + static_cast<int64_t>(threadIdx.x) * stride;
+  if (ok) {
+    const double x = (g.x - cen[0]) * scale, y = (g.y - cen[1]) * scale, z = (g.z - cen[2]) * scale;
+    const float ax = static_cast<float>(g.ax), ay = static_cast<float>(g.ay),
+                az = static_cast<float>(g.az);
+    r[0] = static_cast<float>(x);
+    r[1] = static_cast<float>(y);
+    r[2] = static_cast<float>(z);
+    r[3] = ax;
+    r[4] = ay;
+    r[5] = az;
+    r[6] = static_cast<float>(g.ay * z - g.az * y);
+    r[7] = static_cast<float>(g.az * x - g.ax * z);
+    r[8] = static_cast<float>(g.ax * y - g.ay * x);
+    if (stride == 12) {
+      r[9] = static_cast<float>(g.ax - static_cast<double>(ax));
+      r[10] = static_cast<float>(g.ay - static_cast<double>(ay));
+      r[11] = static_cast<float>(g.az - static_cast<double>(az));
+    }
+  } else {
+    r[0] = r[1] = r[2] = 1e18f;
+    for (int q = 3; q < stride; ++q) r[q] = 0.0f;
+  }
+}
+
+// The far records of one set: workgroup c builds chunk c.
+template <int CH>
+__global__ __launch_bounds__(CH) void far_records_kernel(const Src8* __restrict__ src, int n,
+                                                         int stride, float* __restrict__ rec,
+                                                         double* __restrict__ box) {
+  const int c = blockIdx.x;
+  far_records_chunk<CH>(src, n, c, stride, src[0].s, rec + static_cast<int64_t>(c) * CH * stride,
+                        box + static_cast<int64_t>(c) * 6);
+}
 
 template <typename Real, unsigned OUT, bool UNIF>
 __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const Real* p,
@@ -418,14 +535,28 @@ __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const
     acc[kE + 1] = M::fma(en, s.ay, acc[kE + 1]);
     acc[kE + 2] = M::fma(en, s.az, acc[kE + 2]);
   }
-  if constexpr (S::pse) {  // as in pair_update: alpha_j - alpha_i from high and low parts
+  if constexpr (S::pse) {
+    // sum_j t9 (a_j - a_i) = sum_j t9 a_j - a_i sum_j t9 (the second term
+    // is applied per sub-tile, far_pse_fold).  On a far tile or chunk every
+    // pair is more than 4 radii apart, so t9 = A^-9/2 < 17^-9/2 = 2.9e-6: both
+    // sums are that small beside the near tiles' terms of the same target,
+    // and so are their roundings -- the near tiles keep the differences.
     const Real t9 = t5 * t2 * t2;
-    const Real dax = (s.ax - t.ax) + (slo[0] - t.lx);
-    const Real day = (s.ay - t.ay) + (slo[1] - t.ly);
-    const Real daz = (s.az - t.az) + (slo[2] - t.lz);
-    acc[kP + 0] = M::fma(t9, dax, acc[kP + 0]);
-    acc[kP + 1] = M::fma(t9, day, acc[kP + 1]);
-    acc[kP + 2] = M::fma(t9, daz, acc[kP + 2]);
+    acc[kP + 0] = M::fma(t9, s.ax, acc[kP + 0]);
+    acc[kP + 1] = M::fma(t9, s.ay, acc[kP + 1]);
+    acc[kP + 2] = M::fma(t9, s.az, acc[kP + 2]);
+    acc[kT9] += t9;
+  }
+}
+
+// The far form's PSE partials: sum t9 a_j - a_i sum t9 (see pair_moment).
+template <typename Real, unsigned OUT>
+__device__ __forceinline__ void far_pse_fold(Real* acc, const Tgt<Real>& t) {
+  if constexpr ((OUT & VPM_OUT_PSE) != 0) {
+    acc[kP + 0] = Arith<Real>::fma(-t.ax, acc[kT9], acc[kP + 0]);
+    acc[kP + 1] = Arith<Real>::fma(-t.ay, acc[kT9], acc[kP + 1]);
+    acc[kP + 2] = Arith<Real>::fma(-t.az, acc[kT9], acc[kP + 2]);
+    acc[kT9] = Real(0);
   }
 }
 
@@ -463,12 +594,17 @@ __device__ __forceinline__ bool coherent_wave(const Src8* tg, const bool* valid)
 // split_out == null: writes the outputs of its targets and its enstrophy
 // partial to *ens_slot.  Otherwise writes its raw sums to split_out (Slots::packed layout,
 // leading dimension pld) for the split reduce.
-template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
+// FARS: far chunks of an FP32 pass with the moment form read the far records
+// (far_rec, far_box: this set's, indexed by its sorted source index) through
+// scalar loads; otherwise far tiles are staged in LDS like near ones.
+template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP,
+          bool FARS = false>
 __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, const Src8* src,
                                             const int* perm, const int base, double* ens_slot,
                                             const int lo, const int hi, double* split_out,
-                                            const int64_t pld) {
-  static_assert(TILE % SUB == 0 && TILE % BLOCK == 0, "tile shape");
+                                            const int64_t pld, const float* far_rec = nullptr,
+                                            const double* far_box = nullptr) {
+  static_assert(TILE % SUB == 0 && (TILE % BLOCK == 0 || BLOCK % TILE == 0), "tile shape");
   using S = Slots<OUT>;
   constexpr bool kComp = std::is_same<Real, float>::value && COMP == 0;
   constexpr bool kFold32 = std::is_same<Real, float>::value;
@@ -482,12 +618,18 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   // runs the far-field moment form (pair_moment) on distant tiles.
   constexpr bool kBothG = S::str && (S::pse || (!S::vort && !S::ens));
   constexpr bool kMoment = S::str && S::vel && UNIF;
+  // far chunks from the scalar-loaded far records (kFarS) or far tiles
+  // staged in LDS (kFarL)
+  constexpr bool kFarS = FARS && kMoment && std::is_same<Real, float>::value;
+  constexpr bool kFarL = kMoment && !kFarS;
   __shared__ SrcR<Real, UNIF> lds[TILE];
   __shared__ double lds_a64[S::str ? TILE : 1][3];
   __shared__ float4 lds_lo[kLo ? TILE : 1];
-  __shared__ Real lds_mz[kMoment ? TILE : 1];  // a_j x x_j: (p0, p1) of the record, z here
+  __shared__ Real lds_mz[kFarL ? TILE : 1];  // a_j x x_j: (p0, p1) of the record, z here
   __shared__ double lds_x64[kMoment ? BLOCK * R : 1][3];  // the targets' x_i in FP64
   __shared__ double red[6][BLOCK / 64];
+  __shared__ int wave_coh[BLOCK / 64];
+  __shared__ double wbox[kFarS ? BLOCK / 64 : 1][6];
   __shared__ double centre[3];
   __shared__ double half_ext[3];  // the targets' bounding box, scaled: [-half_ext, half_ext]
 
@@ -515,6 +657,17 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       tg[r] = Src8{0, 0, 0, 0, 0, 0, 1, 0};
     }
   }
+  // G accumulation mode (FP32 passes with stretching): FP64 per pair when a
+  // wave of the workgroup holds coherent strengths.  The choice is made for
+  // the whole workgroup, so that every wave runs the same tile loop and
+  // reaches the same barriers.
+  bool coh = false;
+  if constexpr (kBothG && std::is_same<Real, float>::value)
+    coh = a.g64_mode == VPM_G64_ALWAYS ||
+          (a.g64_mode == VPM_G64_AUTO && coherent_wave<R>(tg, valid));
+  // each wave's own target box (centre, scaled half-extent) in wbox: far
+  // chunks are decided per wave against it, the staging of a tile per
+  // workgroup
   {
     const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
@@ -523,8 +676,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       if (lane == 0) {
         red[c][wave] = lo;
         red[3 + c][wave] = hi;
+        wbox[wave][c] = 0.5 * (lo + hi);
+        wbox[wave][3 + c] = 0.5 * (hi - lo) * scale;
       }
     }
+    if (lane == 0) wave_coh[wave] = coh ? 1 : 0;
     __syncthreads();
     if (tid < 3) {
       double lo = red[tid][0], hi = red[3 + tid][0];
@@ -580,12 +736,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     }
   }
   const Real eps_r2 = static_cast<Real>(eps_r2_d);
-  // Wave-uniform choice of the G accumulation (FP32 passes).
+  // Workgroup-uniform choice of the G accumulation (FP32 passes).
   bool g64 = S::str && std::is_same<Real, float>::value;
   if constexpr (kBothG && std::is_same<Real, float>::value) {
-    g64 = a.g64_mode == VPM_G64_ALWAYS ||
-          (a.g64_mode == VPM_G64_AUTO && coherent_wave<R>(tg, valid));
-    g64 = __builtin_amdgcn_readfirstlane(static_cast<int>(g64)) != 0;
+    int any = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) any |= wave_coh[w];
+    g64 = __builtin_amdgcn_readfirstlane(any) != 0;
   }
 
   Real part[R][kNAcc];
@@ -611,12 +768,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 
   // ---- sweep the source tiles ------------------------------------------
   // One instantiation per G mode (G64: FP64 per-pair G, coherent waves), so
-  // that each loop keeps only its own state live.  The waves of one
-  // workgroup may take different modes and share the staged tile: every
-  // wave stages (and bounds) its share of every LDS array either mode reads.
+  // that each loop keeps only its own state live.  The mode is uniform over
+  // the workgroup, so every barrier below is reached by all of its waves.
   auto sweep = [&](auto mode) {
     constexpr bool G64 = decltype(mode)::value;
-    constexpr bool kFar = kMoment;
+    constexpr bool kFar = kFarL;
     constexpr bool kStageLo = kLo;
     constexpr bool kStageA64 = S::str;
     // FP32 partials of G: the sub-tiles holding this wave's own targets take
@@ -642,96 +798,213 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     // slower; the sets unrolled 1 for occupancy would spill).  FP64 records are twice the registers: the
     // FP64 pass's far loop prefetches fewer of them.
     constexpr int kUnrollFar = std::is_same<Real, float>::value ? (kUnroll == 8 ? 16 : kUnroll) : 2;
+    constexpr int kChunks = TILE / SUB;
+    constexpr unsigned kAllFar = (1u << kChunks) - 1u;
+    static_assert(kChunks <= 32, "far bits");
+    // Chunks of this tile in the far form (kFarS): their box keeps a gap of
+    // more than kFarGapS radii from the targets' box.  Returns the
+    // workgroup's bits (workgroup-uniform: a tile with a near chunk is
+    // staged) and sets the wave's (its own, smaller box: a superset of the
+    // workgroup's bits; wave-uniform).  Chunks at or past hi count as far
+    // and are skipped.
+    auto far_bits = [&](int t0, unsigned* wave_bits) {
+      unsigned bits = 0u, wbits = 0u;
+      auto gap2 = [&](const double* cb, const double* cen, const double* half) {
+        const double gx = fmax(fabs(cb[0] - cen[0]) * scale - half[0] - cb[3], 0.0),
+                     gy = fmax(fabs(cb[1] - cen[1]) * scale - half[1] - cb[4], 0.0),
+                     gz = fmax(fabs(cb[2] - cen[2]) * scale - half[2] - cb[5], 0.0);
+        return gx * gx + gy * gy + gz * gz;
+      };
+      const double wgc[3] = {cx, cy, cz};
+      const double wgh[3] = {half_ext[0], half_ext[1], half_ext[2]};
+#pragma unroll
+      for (int c = 0; c < kChunks; ++c) {
+        const int j0 = t0 + c * SUB;
+        bool f = true, fw = true;
+        if (j0 < hi) {
+          const double* cb = far_box + static_cast<int64_t>(j0 / SUB) * 6;
+          f = gap2(cb, wgc, wgh) > kFarGapS * kFarGapS;
+          fw = f || gap2(cb, &wbox[tid >> 6][0], &wbox[tid >> 6][3]) > kFarGapS * kFarGapS;
+        }
+        bits |= f ? (1u << c) : 0u;
+        wbits |= fw ? (1u << c) : 0u;
+      }
+      *wave_bits = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(wbits)));
+      return static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bits)));
+    };
+    // One far chunk [j0, j0 + SUB) from the far records, in scalar loads.
+    auto far_chunk = [&](int j0) {
+      if constexpr (kFarS) {
+        constexpr int RS = FarRec<OUT>::kStride;
+        const double* cb = far_box + static_cast<int64_t>(j0 / SUB) * 6;
+        const double ox = (cx - cb[0]) * scale, oy = (cy - cb[1]) * scale,
+                     oz = (cz - cb[2]) * scale;
+        Tgt<Real> tc[R];
+        Real pc[R][3];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // the targets relative to the chunk centre
+          tc[r] = t[r];
+          tc[r].x = static_cast<Real>(lds_x64[r * BLOCK + tid][0] + ox);
+          tc[r].y = static_cast<Real>(lds_x64[r * BLOCK + tid][1] + oy);
+          tc[r].z = static_cast<Real>(lds_x64[r * BLOCK + tid][2] + oz);
+          pc[r][0] = tc[r].ay * tc[r].z - tc[r].az * tc[r].y;
+          pc[r][1] = tc[r].az * tc[r].x - tc[r].ax * tc[r].z;
+          pc[r][2] = tc[r].ax * tc[r].y - tc[r].ay * tc[r].x;
+        }
+        const far_float_t* rec = (const far_float_t*)(far_rec + static_cast<int64_t>(j0) * RS);
+        // Sources in groups of kFarGroup, two register buffers: the loads of
+        // the next group issue after the first pair of the current one (whose
+        // wait covers only the current group's loads), so they overlap the
+        // rest of the group's pairs.  Scalar loads return out of order, so a
+        // wait always covers every load in flight; the scheduling barriers
+        // keep the compiler from hoisting the next group's loads above it.
+        constexpr int GR = kFarGroup * RS;
+        static_assert(SUB % (2 * kFarGroup) == 0, "far group");
+        float bA[GR], bB[GR];
+        auto load_group = [&](float* b, int q) {
+#pragma unroll
+          for (int u = 0; u < GR; ++u) b[u] = rec[q * RS + u];
+        };
+        auto pair_of = [&](const float* f) {
+          SrcR<Real, UNIF> sr;
+          sr.x = f[0];
+          sr.y = f[1];
+          sr.z = f[2];
+          sr.ax = f[3];
+          sr.ay = f[4];
+          sr.az = f[5];
+          const Mom<Real> m{f[6], f[7], f[8]};
+          Real slo[3] = {Real(0), Real(0), Real(0)};
+          if constexpr (RS == 12) {
+            slo[0] = f[9];
+            slo[1] = f[10];
+            slo[2] = f[11];
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair_moment<Real, OUT, UNIF>(part[r], tc[r], pc[r], sr, m, slo, kc);
+        };
+        auto group = [&](const float* b, float* nb, int qn) {
+          pair_of(b);
+          __builtin_amdgcn_sched_barrier(0);
+          load_group(nb, qn);  // the last group's prefetch reads the padding
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 1; u < kFarGroup; ++u) pair_of(b + u * RS);
+        };
+        load_group(bA, 0);
+#pragma unroll 1
+        for (int q = 0; q < SUB; q += 2 * kFarGroup) {
+          group(bA, bB, q + kFarGroup);
+          group(bB, bA, q + 2 * kFarGroup);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // u += x_i x G + H of the chunk, in FP64
+          const double X = lds_x64[r * BLOCK + tid][0] + ox, Y = lds_x64[r * BLOCK + tid][1] + oy,
+                       Z = lds_x64[r * BLOCK + tid][2] + oz;
+          const double gx = part[r][kG + 0], gy = part[r][kG + 1], gz = part[r][kG + 2];
+          tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]);
+          tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
+          tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
+          part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
+          far_pse_fold<Real, OUT>(part[r], t[r]);
+        }
+      }
+    };
     for (int t0 = lo; t0 < hi; t0 += TILE) {
-      __syncthreads();
-      Real bmn[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
-      Real bmx[3] = {Real(-INFINITY), Real(-INFINITY), Real(-INFINITY)};
-#pragma unroll
-      for (int k = tid; k < TILE; k += BLOCK) {
-        const int j = t0 + k;
-        SrcR<Real, UNIF> s;
-        if (j < hi) {
-          const Src8 g = src[j];
-          const double x = (g.x - cx) * scale, y = (g.y - cy) * scale, z = (g.z - cz) * scale;
-          s.x = static_cast<Real>(x);
-          s.y = static_cast<Real>(y);
-          s.z = static_cast<Real>(z);
-          s.ax = static_cast<Real>(g.ax);
-          s.ay = static_cast<Real>(g.ay);
-          s.az = static_cast<Real>(g.az);
-          if constexpr (kMoment) {
-            s.p0 = static_cast<Real>(g.ay * z - g.az * y);
-            s.p1 = static_cast<Real>(g.az * x - g.ax * z);
-            lds_mz[k] = static_cast<Real>(g.ax * y - g.ay * x);
-            bmn[0] = s.x < bmn[0] ? s.x : bmn[0]; bmx[0] = s.x > bmx[0] ? s.x : bmx[0];
-            bmn[1] = s.y < bmn[1] ? s.y : bmn[1]; bmx[1] = s.y > bmx[1] ? s.y : bmx[1];
-            bmn[2] = s.z < bmn[2] ? s.z : bmn[2]; bmx[2] = s.z > bmx[2] ? s.z : bmx[2];
-          }
-          if constexpr (kStageA64) {
-            lds_a64[k][0] = g.ax;
-            lds_a64[k][1] = g.ay;
-            lds_a64[k][2] = g.az;
-          }
-          if constexpr (kStageLo)
-            lds_lo[k] = make_float4(static_cast<float>(g.ax - static_cast<double>(s.ax)),
-                                    static_cast<float>(g.ay - static_cast<double>(s.ay)),
-                                    static_cast<float>(g.az - static_cast<double>(s.az)), 0.0f);
-          if constexpr (!UNIF) {
-            s.hs = static_cast<Real>(0.5 * g.s);
-            s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
-            s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
-          }
-        } else {  // padding: far away, zero strength -> contributes exactly 0
-          // (PSE weights alpha_j - alpha_i = -alpha_i by t^9: at this distance
-          // t^9 underflows to 0 in either precision; A stays finite)
-          s.x = s.y = s.z = static_cast<Real>(std::is_same<Real, float>::value ? 1e18 : 1e40);
-          s.ax = s.ay = s.az = Real(0);
-          if constexpr (kStageA64) {
-            lds_a64[k][0] = 0.0;
-            lds_a64[k][1] = 0.0;
-            lds_a64[k][2] = 0.0;
-          }
-          if constexpr (kStageLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          if constexpr (kMoment) {
-            s.p0 = s.p1 = Real(0);
-            lds_mz[k] = Real(0);
-          }
-          if constexpr (!UNIF) {
-            s.hs = Real(0.5);
-            s.is2 = Real(1);
-            s.is3 = Real(1);
-          }
-        }
-        lds[k] = s;
-      }
-      if constexpr (kMoment) {  // the tile's bounding box, per wave
-        const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const Real l = wave_min(bmn[c]), h = wave_max(bmx[c]);
-          if (lane == 0) {
-            red[c][wave] = static_cast<double>(l);
-            red[3 + c][wave] = static_cast<double>(h);
-          }
-        }
-      }
-      __syncthreads();
-      // Moment form for this tile: its gap to the targets exceeds kFarGap.
+      unsigned farbits = 0u, wfarbits = 0u;
+      if constexpr (kFarS) farbits = far_bits(t0, &wfarbits);
+      // a tile whose chunks are all far needs no staging (and no barrier)
       bool far = false;
-      if constexpr (kFar) {
-        double gap2 = 0.0;
+      if (!kFarS || farbits != kAllFar) {
+        __syncthreads();
+        Real bmn[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
+        Real bmx[3] = {Real(-INFINITY), Real(-INFINITY), Real(-INFINITY)};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          double l = red[c][0], h = red[3 + c][0];
-#pragma unroll
-          for (int w = 1; w < BLOCK / 64; ++w) {
-            l = fmin(l, red[c][w]);
-            h = fmax(h, red[3 + c][w]);
+        for (int k = tid; k < TILE; k += BLOCK) {
+          const int j = t0 + k;
+          SrcR<Real, UNIF> s;
+          if (j < hi) {
+            const Src8 g = src[j];
+            const double x = (g.x - cx) * scale, y = (g.y - cy) * scale, z = (g.z - cz) * scale;
+            s.x = static_cast<Real>(x);
+            s.y = static_cast<Real>(y);
+            s.z = static_cast<Real>(z);
+            s.ax = static_cast<Real>(g.ax);
+            s.ay = static_cast<Real>(g.ay);
+            s.az = static_cast<Real>(g.az);
+            if constexpr (kFarL) {
+              s.p0 = static_cast<Real>(g.ay * z - g.az * y);
+              s.p1 = static_cast<Real>(g.az * x - g.ax * z);
+              lds_mz[k] = static_cast<Real>(g.ax * y - g.ay * x);
+              bmn[0] = s.x < bmn[0] ? s.x : bmn[0]; bmx[0] = s.x > bmx[0] ? s.x : bmx[0];
+              bmn[1] = s.y < bmn[1] ? s.y : bmn[1]; bmx[1] = s.y > bmx[1] ? s.y : bmx[1];
+              bmn[2] = s.z < bmn[2] ? s.z : bmn[2]; bmx[2] = s.z > bmx[2] ? s.z : bmx[2];
+            }
+            if constexpr (kStageA64) {
+              lds_a64[k][0] = g.ax;
+              lds_a64[k][1] = g.ay;
+              lds_a64[k][2] = g.az;
+            }
+            if constexpr (kStageLo)
+              lds_lo[k] = make_float4(static_cast<float>(g.ax - static_cast<double>(s.ax)),
+                                      static_cast<float>(g.ay - static_cast<double>(s.ay)),
+                                      static_cast<float>(g.az - static_cast<double>(s.az)), 0.0f);
+            if constexpr (!UNIF) {
+              s.hs = static_cast<Real>(0.5 * g.s);
+              s.is2 = static_cast<Real>(1.0 / (g.s * g.s));
+              s.is3 = static_cast<Real>(1.0 / (g.s * g.s * g.s));
+            }
+          } else {  // padding: far away, zero strength -> contributes exactly 0
+            // (PSE weights alpha_j - alpha_i = -alpha_i by t^9: at this distance
+            // t^9 underflows to 0 in either precision; A stays finite)
+            s.x = s.y = s.z = static_cast<Real>(std::is_same<Real, float>::value ? 1e18 : 1e40);
+            s.ax = s.ay = s.az = Real(0);
+            if constexpr (kStageA64) {
+              lds_a64[k][0] = 0.0;
+              lds_a64[k][1] = 0.0;
+              lds_a64[k][2] = 0.0;
+            }
+            if constexpr (kStageLo) lds_lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if constexpr (kFarL) {
+              s.p0 = s.p1 = Real(0);
+              lds_mz[k] = Real(0);
+            }
+            if constexpr (!UNIF) {
+              s.hs = Real(0.5);
+              s.is2 = Real(1);
+              s.is3 = Real(1);
+            }
           }
-          const double gp = fmax(fmax(l - half_ext[c], -half_ext[c] - h), 0.0);
-          gap2 += gp * gp;
+          lds[k] = s;
         }
-        far = __builtin_amdgcn_readfirstlane(static_cast<int>(gap2 > kFarGap * kFarGap)) != 0;
+        if constexpr (kFarL) {  // the tile's bounding box, per wave
+          const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const Real l = wave_min(bmn[c]), h = wave_max(bmx[c]);
+            if (lane == 0) {
+              red[c][wave] = static_cast<double>(l);
+              red[3 + c][wave] = static_cast<double>(h);
+            }
+          }
+        }
+        __syncthreads();
+        // Moment form for this tile: its gap to the targets exceeds kFarGap.
+        if constexpr (kFar) {
+          double gap2 = 0.0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            double l = red[c][0], h = red[3 + c][0];
+#pragma unroll
+            for (int w = 1; w < BLOCK / 64; ++w) {
+              l = fmin(l, red[c][w]);
+              h = fmax(h, red[3 + c][w]);
+            }
+            const double gp = fmax(fmax(l - half_ext[c], -half_ext[c] - h), 0.0);
+            gap2 += gp * gp;
+          }
+          far = __builtin_amdgcn_readfirstlane(static_cast<int>(gap2 > kFarGap * kFarGap)) != 0;
+        }
       }
 
       // Folds the sub-tile's FP32 partials: G into its FP64 totals, the others
@@ -801,6 +1074,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
               tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
               tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
               part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
+              far_pse_fold<Real, OUT>(part[r], t[r]);
             }
           }
           fold_sub(std::integral_constant<bool, kFar>{});
@@ -808,6 +1082,15 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       } else {
 #pragma unroll 1
         for (int sb = 0; sb < TILE; sb += SUB) {
+          if constexpr (kFarS) {
+            if ((wfarbits >> (sb / SUB)) & 1u) {
+              if (t0 + sb < hi) {
+                far_chunk(t0 + sb);
+                fold_sub(std::true_type{});
+              }
+              continue;
+            }
+          }
           if constexpr (G64) {
 #pragma unroll kUnroll
             for (int q = 0; q < SUB; ++q) {
@@ -948,29 +1231,50 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 // Minimum waves per SIMD the register allocation must allow.  The
 // velocity/stretching FP32 pass with 512 lanes x one target holds three
 // loops (FP64 G, FP32 G near, moment form far); left alone the allocator
-// settles just above 128 VGPRs, which would halve the workgroups per CU
-// (two of 512 lanes fit the LDS; smaller workgroups are LDS-bound anyway).
-template <typename Real, unsigned OUT, int BLOCK, int R, int TILE>
+// settles just above 128 VGPRs, which would halve the workgroups per CU.
+// With the far form in scalar-loaded records (FARS) the single-set kernel
+// fits 80 VGPRs: six waves per SIMD, three workgroups of 512 lanes per CU
+// (random cloud of 2^20: 638 ms against 663 ms at four waves and 659 ms at
+// five; the near FP64-G loop then keeps a few values in scratch, rings
+// unchanged).
+#ifndef VPM_W256
+#define VPM_W256 3
+#endif
+template <typename Real, unsigned OUT, int BLOCK, int R, int TILE, bool FARS = false>
 constexpr int min_waves() {
   using S = Slots<OUT>;
   if (!std::is_same<Real, float>::value || R != 1 || TILE > 512 || !S::str) return 1;
-  if (BLOCK == 512) return (!S::vort && !S::ens) ? 4 : 1;
+  // 512 lanes with far records: six waves per SIMD (three workgroups fit the
+  // LDS of 512-source tiles), eight with 256-source tiles
+  if (BLOCK == 512) return (!S::vort && !S::ens) ? (FARS ? (TILE <= 256 ? 8 : 6) : 4) : 1;
   // PSE and LES sets of 256 lanes: three workgroups per CU
-  if (BLOCK == 256) return (S::pse || S::vort || S::ens) ? 3 : 1;
+  if (BLOCK == 256) return (S::pse || S::vort || S::ens) ? VPM_W256 : 1;
   return 1;
+}
+
+// Single-set FP32 passes with the moment form and chunks of 64 or more
+// sources read far records (shapes with 32-source sub-tiles stage far tiles
+// in LDS).
+template <typename Real, unsigned OUT, bool UNIF, int SUB>
+constexpr bool uses_far_records() {
+  return std::is_same<Real, float>::value && UNIF && (OUT & VPM_OUT_VEL) &&
+         (OUT & VPM_OUT_STRETCH) && SUB >= 64;
 }
 
 // grid (target blocks, splits)
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE>())) void fused_kernel(
+__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE,
+                                               uses_far_records<Real, OUT, UNIF, SUB>()>()))
+void fused_kernel(
     FusedArgs a) {
   const int k = static_cast<int>(blockIdx.y);
   const int lo = k * a.chunk;
   const int hi = a.chunk < a.n - lo ? lo + a.chunk : a.n;
   double* part = a.part ? a.part + static_cast<int64_t>(k) * Slots<OUT>::nslot * a.pld : nullptr;
-  fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
+  fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP,
+              uses_far_records<Real, OUT, UNIF, SUB>()>(
       a, a.n, a.src, a.perm, static_cast<int>(blockIdx.x) * (BLOCK * R), a.ens_part + blockIdx.x,
-      lo, hi, part, a.pld);
+      lo, hi, part, a.pld, a.far_rec, a.far_box);
 }
 
 // Adds the splits' raw sums in split order (deterministic) and stores; one
@@ -1014,7 +1318,7 @@ struct TileCfg {
 constexpr TileCfg kCfgs[] = {
     {512, 1, 512, 64, 0},  {256, 1, 512, 64, 0},  {256, 2, 256, 32, 1}, {256, 1, 256, 64, 0},
     {256, 1, 256, 64, 1},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
-    {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},
+    {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},  {512, 1, 256, 128, 0},
 };
 
 constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
@@ -1083,6 +1387,14 @@ int launch_cfg(const FusedArgs& a0, hipStream_t st, int* nblocks) {
     a.chunk = a.n;
     a.pld = 0;
   }
+  if constexpr (uses_far_records<Real, OUT, UNIF, SUB>()) {
+    if (!a.far_rec || !a.far_box) return fail(VPM_ERR_WORKSPACE, "fused pass: no far records");
+    static_assert(SUB == kMinFarChunk || SUB == kMaxFarChunk, "far chunk");
+    hipLaunchKernelGGL(far_records_kernel<SUB>, dim3(static_cast<unsigned>(ceil_div(a.n, SUB))),
+                       dim3(SUB), 0, st, a.src, a.n, FarRec<OUT>::kStride,
+                       const_cast<float*>(a.far_rec), const_cast<double*>(a.far_box));
+    VPM_LAUNCH_CHECK("far_records_kernel");
+  }
   hipLaunchKernelGGL(kern, dim3(blocks, ks), dim3(BLOCK), 0, st, a);
   VPM_LAUNCH_CHECK("fused_kernel");
   if (ks > 1) {
@@ -1108,6 +1420,7 @@ int launch_f32(const FusedArgs& a, int cfg, hipStream_t st, int* nb) {
   if constexpr (UNIF && hot_mask(OUT)) {
     switch (cfg) {
       VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7) VPM_CFG(8) VPM_CFG(9)
+      VPM_CFG(10)
       default:
         break;
     }
@@ -1156,9 +1469,18 @@ int dispatch(const FusedArgs& a, unsigned mask, int prec, bool unif, int cfg, hi
 // ------------------------------------------------------------------ workspace
 struct WsLayout {
   size_t bbox_part, bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, scratch_vec, split,
-      cub, total;
+      far_rec, far_box, cub, total;
   size_t cub_bytes, split_bytes;
 };
+
+// Far records of an FP32 pass with a uniform radius and velocity +
+// stretching: records for whole chunks of the largest size (widest record),
+// boxes for chunks of the smallest.
+bool wants_far_records(const vpm_forward_opts_t* o) {
+  if (!o || o->precision != VPM_PREC_F32C || o->sigma_mode != VPM_SIGMA_UNIFORM) return false;
+  const unsigned m = kernel_mask(o->outputs);  // the instantiation that runs
+  return (m & VPM_OUT_VEL) && (m & VPM_OUT_STRETCH);
+}
 
 // FP64 partial slots per target of a source split (see Slots::nslot)
 int split_slots(unsigned mask) {
@@ -1186,7 +1508,10 @@ size_t split_capacity(int64_t n, const vpm_forward_opts_t* o) {
 
 // The hipCUB scratch comes last, so the other offsets do not depend on it;
 // calls that do not sort pass need_cub = false and skip the size query.
-int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = true) {
+// split_cap: at most this many bytes for the source-split partials (the
+// launch then picks fewer splits).
+int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = true,
+           size_t split_cap = SIZE_MAX) {
   const size_t N = static_cast<size_t>(n > 0 ? n : 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -1204,8 +1529,15 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = 
   L->ens_part = take((N / 64 + 2) * sizeof(double));
   // Unrequested outputs of a superset kernel land here: 3 vectors of N.
   L->scratch_vec = take(3 * N * sizeof(double));
-  L->split_bytes = split_capacity(n, o);
+  L->split_bytes = std::min(split_capacity(n, o), split_cap);
   L->split = take(L->split_bytes);
+  const bool far = wants_far_records(o);
+  const int64_t nn = n > 0 ? n : 1;
+  // + one group of records read ahead past the last chunk
+  L->far_rec = take(far ? (static_cast<size_t>(ceil_div(nn, kMaxFarChunk)) * kMaxFarChunk +
+                           kFarGroup) * 12 * sizeof(float)
+                        : 0);
+  L->far_box = take(far ? static_cast<size_t>(ceil_div(nn, kMinFarChunk)) * 6 * sizeof(double) : 0);
   size_t cub_bytes = 0;
   if (need_cub && (!o || !o->presorted)) {
     const int rc = sort_temp_bytes(n, &cub_bytes);
@@ -1225,19 +1557,25 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = 
 // the host lists them largest set first and workgroups pull them from a
 // counter, so the long items start first and the tail is short.
 struct BatchArgs {
-  FusedArgs f;            // outputs in caller order; f.ens_part: one slot per item
+  FusedArgs f;            // outputs in caller order; f.ens_part: one slot per item;
+                          // f.far_rec / f.far_box: every set's far records
   const Src8* src;        // sorted, concatenated
   const int* perm;        // sorted position -> caller index
   const int* inst_off;    // [S+1]
   const int* item_inst;   // [items]
   const int* item_block;  // [items]
+  const int* set_chunk0;  // [S+1] first far-record chunk of each set (SUB-source chunks)
+  int nsets;
+  int n_total;            // particles over all sets
   int n_items;
   int* counter;
 };
 
 template <typename Real, unsigned OUT, bool UNIF, int BLOCK, int R, int TILE, int SUB, int COMP>
-__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE>())) void fused_batch_kernel(
-    BatchArgs b) {
+__global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE,
+                                               uses_far_records<Real, OUT, UNIF, SUB>()>()))
+void fused_batch_kernel(BatchArgs b) {
+  constexpr bool kFarS = uses_far_records<Real, OUT, UNIF, SUB>();
   __shared__ int item_s;
   for (;;) {
     __syncthreads();
@@ -1249,10 +1587,67 @@ __global__ __launch_bounds__(BLOCK, (min_waves<Real, OUT, BLOCK, R, TILE>())) vo
     const int inst = b.item_inst[item];
     const int off = b.inst_off[inst];
     const int n = b.inst_off[inst + 1] - off;
-    fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP>(
+    const int64_t c0 = kFarS ? b.set_chunk0[inst] : 0;
+    fused_block<Real, OUT, UNIF, BLOCK, R, TILE, SUB, COMP, kFarS>(
         b.f, n, b.src + off, b.perm + off, b.item_block[item] * (BLOCK * R), b.f.ens_part + item,
-        0, n, nullptr, 0);
+        0, n, nullptr, 0, kFarS ? b.f.far_rec + c0 * SUB * FarRec<OUT>::kStride : nullptr,
+        kFarS ? b.f.far_box + c0 * 6 : nullptr);
   }
+}
+
+// set_chunk0[s] = sum over sets t < s of ceil(n_t / ch); set_chunk0[S] = the
+// total.  One workgroup: each thread sums a contiguous range of sets, then a
+// scan of the per-thread sums in LDS.
+__global__ __launch_bounds__(1024) void set_chunks_kernel(const int* __restrict__ off, int nsets,
+                                                          int ch, int* __restrict__ chunk0) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nsets + 1023) / 1024;
+  const int b = t * per, e = min(nsets, b + per);
+  int sum = 0;
+  for (int s = b; s < e; ++s) sum += (off[s + 1] - off[s] + ch - 1) / ch;
+  part[t] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const int v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int acc = part[t];
+  for (int s = b; s < e; ++s) {
+    chunk0[s] = acc;
+    acc += (off[s + 1] - off[s] + ch - 1) / ch;
+  }
+  if (t == 1023) chunk0[nsets] = acc;
+}
+
+// The far records of every set: workgroup g builds chunk g (of all sets'
+// chunks; the owning set by binary search in set_chunk0), like
+// far_records_kernel for one set.
+template <int CH>
+__global__ __launch_bounds__(CH) void set_far_records_kernel(const Src8* __restrict__ src,
+                                                             const int* __restrict__ off,
+                                                             const int* __restrict__ chunk0,
+                                                             int nsets, int stride,
+                                                             float* __restrict__ rec,
+                                                             double* __restrict__ box) {
+  const int g = blockIdx.x;
+  if (g >= chunk0[nsets]) return;  // workgroup-uniform: no barrier is skipped by part of it
+  int lo = 0, hi = nsets - 1;      // the set s with chunk0[s] <= g < chunk0[s+1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (chunk0[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int s = lo;
+  const int n = off[s + 1] - off[s];
+  const int c = g - chunk0[s];
+  far_records_chunk<CH>(src + off[s], n, c, stride, src[off[s]].s, rec + (int64_t)g * CH * stride,
+                        box + (int64_t)g * 6);
 }
 
 __global__ void counter_reset_kernel(int* c) {
@@ -1400,6 +1795,21 @@ int launch_batch_cfg(const BatchArgs& b, int grid, hipStream_t st) {
     grid = std::max(1, per_cu) * std::max(1, cus);
   }
   grid = std::min(grid, b.n_items);
+  if constexpr (uses_far_records<Real, OUT, true, SUB>()) {
+    static_assert(SUB == kMinFarChunk || SUB == kMaxFarChunk, "far chunk");
+    if (!b.f.far_rec || !b.f.far_box || !b.set_chunk0)
+      return fail(VPM_ERR_WORKSPACE, "batch pass: no far records");
+    int* chunk0 = const_cast<int*>(b.set_chunk0);
+    hipLaunchKernelGGL(set_chunks_kernel, dim3(1), dim3(1024), 0, st, b.inst_off, b.nsets, SUB,
+                       chunk0);
+    VPM_LAUNCH_CHECK("set_chunks_kernel");
+    const int64_t max_chunks = ceil_div(b.n_total, SUB) + b.nsets;  // bound; extra groups exit
+    hipLaunchKernelGGL(set_far_records_kernel<SUB>, dim3(static_cast<unsigned>(max_chunks)),
+                       dim3(SUB), 0, st, b.src, b.inst_off, b.set_chunk0, b.nsets,
+                       FarRec<OUT>::kStride, const_cast<float*>(b.f.far_rec),
+                       const_cast<double*>(b.f.far_box));
+    VPM_LAUNCH_CHECK("set_far_records_kernel");
+  }
   hipLaunchKernelGGL((fused_batch_kernel<Real, OUT, true, BLOCK, R, TILE, SUB, COMP>), dim3(grid),
                      dim3(BLOCK), 0, st, b);
   VPM_LAUNCH_CHECK("fused_batch_kernel");
@@ -1417,7 +1827,7 @@ int launch_batch_mask(const BatchArgs& b, int prec, int cfg, int grid, hipStream
                             kCfgs[I].sub, kCfgs[I].comp>(b, grid, st);
   switch (cfg) {
     VPM_BCFG(1) VPM_BCFG(2) VPM_BCFG(3) VPM_BCFG(4) VPM_BCFG(5) VPM_BCFG(6) VPM_BCFG(7)
-    VPM_BCFG(8) VPM_BCFG(9)
+    VPM_BCFG(8) VPM_BCFG(9) VPM_BCFG(10)
     default:
       break;
   }
@@ -1443,8 +1853,8 @@ unsigned batch_mask(unsigned want) {
 }
 
 struct BatchLayout {
-  size_t bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, cand, scratch_vec, counter, cub,
-      total, cub_bytes;
+  size_t bbox, keys_in, keys_out, idx_in, perm, src8, ens_part, cand, scratch_vec, counter,
+      set_chunk0, far_rec, far_box, cub, total, cub_bytes;
 };
 
 int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L, bool need_cub = true) {
@@ -1465,6 +1875,12 @@ int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L, bool need_cu
   L->cand = take(N * sizeof(double));
   L->scratch_vec = take(3 * N * sizeof(double));
   L->counter = take(64);
+  // far records: every set's chunks of 64 or 128 sources, at most one partial
+  // chunk per set, plus one group read ahead past the last
+  const size_t S = static_cast<size_t>(nsets > 0 ? nsets : 1);
+  L->set_chunk0 = take((S + 1) * sizeof(int));
+  L->far_rec = take((N + kMaxFarChunk * S + kFarGroup) * 12 * sizeof(float));
+  L->far_box = take((N / kMinFarChunk + S + 1) * 6 * sizeof(double));
   size_t cb = 0;
   if (need_cub) {
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(
@@ -1583,25 +1999,23 @@ int check_ws(int64_t n, const vpm_forward_opts_t* o, void* ws, size_t ws_bytes, 
 
 }  // namespace
 
-int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, void* workspace,
-                        size_t ws_bytes, void* stream) {
-  clear_error();
-  int rc = check_forward(p, o, "vpm_forward_prepare");
-  if (rc) return rc;
+namespace {
+
+// The sort and gather of a forward pass into a workspace laid out as L.
+int forward_prepare_impl(const vpm_particles_t* p, const vpm_forward_opts_t* o, char* ws,
+                         const WsLayout& L, hipStream_t st) {
   const int64_t n = p->n;
-  WsLayout L{};
-  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_prepare", true))) return rc;
   if (n == 0) return VPM_OK;
-  hipStream_t st = as_stream(stream);
-  char* ws = static_cast<char*>(workspace);
   const int ni = static_cast<int>(n);
   int* perm = nullptr;
   if (!o->presorted) {
     perm = reinterpret_cast<int*>(ws + L.perm);
-    rc = morton_sort(p, reinterpret_cast<double*>(ws + L.bbox_part),
-                     reinterpret_cast<double*>(ws + L.bbox), reinterpret_cast<uint32_t*>(ws + L.keys_in),
-                     reinterpret_cast<uint32_t*>(ws + L.keys_out), reinterpret_cast<int*>(ws + L.idx_in),
-                     perm, ws + L.cub, L.cub_bytes, st);
+    const int rc = morton_sort(p, reinterpret_cast<double*>(ws + L.bbox_part),
+                               reinterpret_cast<double*>(ws + L.bbox),
+                               reinterpret_cast<uint32_t*>(ws + L.keys_in),
+                               reinterpret_cast<uint32_t*>(ws + L.keys_out),
+                               reinterpret_cast<int*>(ws + L.idx_in), perm, ws + L.cub,
+                               L.cub_bytes, st);
     if (rc) return rc;
   }
   Src8* src8 = reinterpret_cast<Src8*>(ws + L.src8);
@@ -1611,24 +2025,33 @@ int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, v
   return VPM_OK;
 }
 
-int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,
-                    void* workspace, size_t ws_bytes, void* stream) {
-  clear_error();
-  int rc = check_forward(p, o, "vpm_forward_run");
-  if (rc) return rc;
-  if (!out) return fail(VPM_ERR_ARG, "vpm_forward_run: null out");
+int check_forward_out(const vpm_particles_t* p, const vpm_forward_opts_t* o,
+                      const vpm_forward_out_t* out, const char* who) {
+  if (!out) {
+    set_error("%s: null out", who);
+    return VPM_ERR_ARG;
+  }
   const unsigned want = o->outputs;
   const int64_t n = p->n;
-  if (out->ld < n) return fail(VPM_ERR_ARG, "vpm_forward_run: out->ld < n");
-  if ((want & VPM_OUT_VEL) && n && !out->vel) return fail(VPM_ERR_ARG, "vpm_forward_run: vel output missing");
-  if ((want & VPM_OUT_STRETCH) && n && !out->stretch) return fail(VPM_ERR_ARG, "vpm_forward_run: stretch output missing");
-  if ((want & VPM_OUT_PSE) && n && !out->pse) return fail(VPM_ERR_ARG, "vpm_forward_run: pse output missing");
-  if ((want & VPM_OUT_VORT) && n && !out->vort) return fail(VPM_ERR_ARG, "vpm_forward_run: vort output missing");
-  if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) return fail(VPM_ERR_ARG, "vpm_forward_run: enstrophy output missing");
-  WsLayout L{};
-  if ((rc = check_ws(n, o, workspace, ws_bytes, &L, "vpm_forward_run", false))) return rc;
-  hipStream_t st = as_stream(stream);
-  char* ws = static_cast<char*>(workspace);
+  const char* miss = nullptr;
+  if (out->ld < n) miss = "out->ld < n";
+  else if ((want & VPM_OUT_VEL) && n && !out->vel) miss = "vel output missing";
+  else if ((want & VPM_OUT_STRETCH) && n && !out->stretch) miss = "stretch output missing";
+  else if ((want & VPM_OUT_PSE) && n && !out->pse) miss = "pse output missing";
+  else if ((want & VPM_OUT_VORT) && n && !out->vort) miss = "vort output missing";
+  else if ((want & VPM_OUT_ENSTROPHY) && !out->enstrophy) miss = "enstrophy output missing";
+  if (miss) {
+    set_error("%s: %s", who, miss);
+    return VPM_ERR_ARG;
+  }
+  return VPM_OK;
+}
+
+// The pass itself, from the sorted records of forward_prepare_impl.
+int forward_run_impl(const vpm_particles_t* p, const vpm_forward_opts_t* o,
+                     vpm_forward_out_t* out, char* ws, const WsLayout& L, hipStream_t st) {
+  const unsigned want = o->outputs;
+  const int64_t n = p->n;
   if (n == 0) {
     if (want & VPM_OUT_ENSTROPHY) {
       hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, st, out->enstrophy, int64_t(1), 0.0);
@@ -1662,11 +2085,14 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   a.part_cap = L.split_bytes;
   a.max_splits = o->max_splits;
   a.g64_mode = o->g64_mode;
+  if (wants_far_records(o)) {
+    a.far_rec = reinterpret_cast<const float*>(ws + L.far_rec);
+    a.far_box = reinterpret_cast<const double*>(ws + L.far_box);
+  }
   int nblocks = 0;
   const int cfg = effective_config(mask, o->sigma_mode == VPM_SIGMA_UNIFORM, o->tile_config);
-  if ((rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st,
-                     &nblocks)))
-    return rc;
+  int rc = dispatch(a, mask, o->precision, o->sigma_mode == VPM_SIGMA_UNIFORM, cfg, st, &nblocks);
+  if (rc) return rc;
   if (want & VPM_OUT_ENSTROPHY) {
     const double k = (1.0 / (8.0 * kPi)) * kOneOverFourPi;
     hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(1024), 0, st, a.ens_part,
@@ -1674,6 +2100,29 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
     VPM_LAUNCH_CHECK("final_sum_kernel");
   }
   return VPM_OK;
+}
+
+}  // namespace
+
+int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* o, void* workspace,
+                        size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_forward(p, o, "vpm_forward_prepare");
+  if (rc) return rc;
+  WsLayout L{};
+  if ((rc = check_ws(p->n, o, workspace, ws_bytes, &L, "vpm_forward_prepare", true))) return rc;
+  return forward_prepare_impl(p, o, static_cast<char*>(workspace), L, as_stream(stream));
+}
+
+int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  clear_error();
+  int rc = check_forward(p, o, "vpm_forward_run");
+  if (rc) return rc;
+  if ((rc = check_forward_out(p, o, out, "vpm_forward_run"))) return rc;
+  WsLayout L{};
+  if ((rc = check_ws(p->n, o, workspace, ws_bytes, &L, "vpm_forward_run", false))) return rc;
+  return forward_run_impl(p, o, out, static_cast<char*>(workspace), L, as_stream(stream));
 }
 
 int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,
@@ -1838,6 +2287,11 @@ int vpm_batch_run(const vpm_particles_t* p, const vpm_batch_desc_t* d, const vpm
   b.inst_off = d->set_off;
   b.item_inst = d->item_set;
   b.item_block = d->item_block;
+  b.set_chunk0 = reinterpret_cast<const int*>(ws + L.set_chunk0);
+  b.f.far_rec = reinterpret_cast<const float*>(ws + L.far_rec);
+  b.f.far_box = reinterpret_cast<const double*>(ws + L.far_box);
+  b.nsets = d->nsets;
+  b.n_total = static_cast<int>(n);
   b.n_items = d->n_items;
   b.counter = reinterpret_cast<int*>(ws + L.counter);
   if ((rc = dispatch_batch(b, batch_mask(want), o->precision,

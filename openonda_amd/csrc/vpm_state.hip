@@ -229,6 +229,33 @@ __global__ __launch_bounds__(kB) void les_commit_kernel(int n, const double* __r
   if (i < n && vmax[0] > 1e-9) nu_t[i] = cand[i];
 }
 
+// Particle-set scan: ORs VPM_BAD_* bits into *flags for non-finite
+// positions or strengths, radii that are not finite and > 0, viscosities
+// that are not finite and >= 0.  One flag write per wave that saw a problem.
+__global__ __launch_bounds__(kB) void scan_kernel(int n, int64_t ld, const double* __restrict__ pos,
+                                                  const double* __restrict__ str,
+                                                  const double* __restrict__ sigma,
+                                                  const double* __restrict__ nu,
+                                                  unsigned* __restrict__ flags) {
+  unsigned bad = 0u;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (!isfinite(pos[k * ld + i])) bad |= VPM_BAD_POSITION;
+      if (!isfinite(str[k * ld + i])) bad |= VPM_BAD_STRENGTH;
+    }
+    const double s = sigma[i];
+    if (!(isfinite(s) && s > 0.0)) bad |= VPM_BAD_RADIUS;
+    if (nu) {
+      const double v = nu[i];
+      if (!(isfinite(v) && v >= 0.0)) bad |= VPM_BAD_VISCOSITY;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad |= __shfl_xor(bad, off, 64);
+  if ((threadIdx.x & 63) == 0 && bad) atomicOr(flags, bad);
+}
+
 inline int blocks_reduce(int64_t n) {
   return static_cast<int>(std::min<int64_t>(1024, ceil_div(n > 0 ? n : 1, kB)));
 }
@@ -301,6 +328,19 @@ int vpm_relax_strengths(int64_t n, int64_t ld, double* strengths, const double* 
   hipLaunchKernelGGL(relax_kernel, dim3(grid_for(n)), dim3(kB), 0, as_stream(stream),
                      static_cast<int>(n), ld, strengths, vorticity);
   VPM_LAUNCH_CHECK("relax_kernel");
+  return VPM_OK;
+}
+
+int vpm_scan_particles(const vpm_particles_t* p, uint32_t* flags, void* stream) {
+  clear_error();
+  int rc = check_particles(p, false, "vpm_scan_particles");
+  if (rc) return rc;
+  if (!flags) return fail(VPM_ERR_ARG, "vpm_scan_particles: null flags");
+  if (p->n == 0) return VPM_OK;
+  hipLaunchKernelGGL(scan_kernel, dim3(blocks_reduce(p->n)), dim3(kB), 0, as_stream(stream),
+                     static_cast<int>(p->n), p->ld, p->pos, p->str, p->sigma, p->nu,
+                     reinterpret_cast<unsigned*>(flags));
+  VPM_LAUNCH_CHECK("scan_kernel");
   return VPM_OK;
 }
 

@@ -191,6 +191,24 @@ def total_enstrophy(pos, alpha, sigma, variant="cuda"):
 
 
 # ------------------------------------------------------------ fused forward
+def scan_particles(pos, alpha, sigma, nu=None) -> int:
+    """VPM_BAD_* bits of a device particle set (0: every position and
+    strength finite, every radius finite and > 0, every viscosity finite and
+    >= 0).  One scan kernel and a 4-byte read back (synchronises the stream)."""
+    p = _particles(pos, alpha, sigma, nu)
+    flags = torch.zeros(1, dtype=torch.int32, device=pos.device)
+    L.call("vpm_scan_particles", C.byref(p), flags.data_ptr(), _stream(pos))
+    return int(flags.item())
+
+
+def check_particles(pos, alpha, sigma, nu=None, where="particle set"):
+    """Raises NonFiniteError (status VPM_ERR_NONFINITE) when
+    :func:`scan_particles` finds anything."""
+    bits = scan_particles(pos, alpha, sigma, nu)
+    if bits:
+        raise L.NonFiniteError(where, bits)
+
+
 def sigma_is_uniform(sigma: torch.Tensor) -> bool:
     """True when every radius equals sigma[0] (synchronises the stream)."""
     return sigma.numel() == 0 or bool((sigma == sigma[0]).all())

@@ -10,6 +10,11 @@
     B / 1024 / (counter of the copy), applied to the kernel's reading.  The
     result is written as a table and a fenced JSON block that bench.py reads
     (``traffic_bytes`` for the workload named by W).
+``python -m openonda_amd.profiling valu DB... --kernel K --workload W --pairs P
+        --flops-per-pair F --kernel-ms T --title T --command C``
+    VALU counters of kernel K from one or more --pmc passes: instructions per
+    pair by class, executed FP32 TFLOP/s against the vector peak beside the
+    algorithmic figure, dual-issue share, wait shares; JSON block as above.
 """
 from __future__ import annotations
 
@@ -103,6 +108,99 @@ def traffic_markdown(fetch_db, write_db, kernel, copy_bytes, workload, title, co
     return "\n".join(lines) + "\n"
 
 
+def kernel_counters(dbs, kernel: str):
+    """{counter: value} of the last dispatch of ``kernel`` in each database
+    (one rocprofv3 --pmc pass per database)."""
+    vals = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        names = [r[0] for r in con.execute("select distinct counter_name from counters_collection")]
+        con.close()
+        for c in names:
+            hits = [v for _, k, v in counter_values(db, c) if kernel in k]
+            if hits:
+                vals[c] = hits[-1]
+    return vals
+
+
+# FP32 flops per lane of each counted VALU instruction class (FMA 2)
+_FP32 = {"SQ_INSTS_VALU_FMA_F32": 2, "SQ_INSTS_VALU_MUL_F32": 1, "SQ_INSTS_VALU_ADD_F32": 1,
+         "SQ_INSTS_VALU_TRANS_F32": 1}
+_FP64 = {"SQ_INSTS_VALU_FMA_F64": 2, "SQ_INSTS_VALU_MUL_F64": 1, "SQ_INSTS_VALU_ADD_F64": 1}
+
+
+def valu_summary(vals, pairs, flops_per_pair, kernel_ms, cus=256, peak_tflops=157.3):
+    """Executed-instruction figures of one launch from its counters: VALU
+    instructions per pair (wave instructions x 64 lanes / pairs), FP32 and
+    FP64 flops per pair by instruction class, executed FP32 TFLOP/s against
+    the vector peak beside the algorithmic figure, and issue statistics."""
+    lanes = 64.0
+    out = {"pairs": pairs, "kernel_ms": kernel_ms}
+    per_pair = lambda c: vals[c] * lanes / pairs if c in vals else None
+    for c in ("SQ_INSTS_VALU", *_FP32, *_FP64, "SQ_INSTS_VALU_CVT", "SQ_INSTS_LDS",
+              "SQ_INSTS_SMEM", "SQ_INSTS_SALU"):
+        if c in vals:
+            out[c + "_per_pair"] = per_pair(c)
+    if all(c in vals for c in _FP32):
+        f32 = sum(k * vals[c] for c, k in _FP32.items()) * lanes
+        out["exec_fp32_flop_per_pair"] = f32 / pairs
+        out["exec_fp32_tflops"] = f32 / (kernel_ms * 1e-3) / 1e12
+        out["exec_fp32_frac"] = out["exec_fp32_tflops"] / peak_tflops
+    if all(c in vals for c in _FP64):
+        f64 = sum(k * vals[c] for c, k in _FP64.items()) * lanes
+        out["exec_fp64_flop_per_pair"] = f64 / pairs
+    out["alg_tflops"] = flops_per_pair * pairs / (kernel_ms * 1e-3) / 1e12
+    out["alg_frac"] = out["alg_tflops"] / peak_tflops
+    if "SQ_INSTS_VALU" in vals:
+        clk = kernel_ms * 1e-3 * 2.4e9 * cus
+        out["valu_wave_instr_per_clk_per_cu"] = vals["SQ_INSTS_VALU"] / clk
+    if "SQ_ACTIVE_INST_VALU" in vals and "SQ_ACTIVE_INST_VALU2" in vals and vals["SQ_ACTIVE_INST_VALU"]:
+        out["dual_issue_share"] = vals["SQ_ACTIVE_INST_VALU2"] / vals["SQ_ACTIVE_INST_VALU"]
+    if "SQ_ACTIVE_INST_VALU" in vals and "GRBM_GUI_ACTIVE" in vals and vals["GRBM_GUI_ACTIVE"]:
+        out["valu_busy"] = vals["SQ_ACTIVE_INST_VALU"] / cus / vals["GRBM_GUI_ACTIVE"]
+    if "SQ_WAIT_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
+        out["wait_any_share"] = vals["SQ_WAIT_ANY"] / vals["SQ_WAVE_CYCLES"]
+    if "SQ_WAIT_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
+        out["wait_issue_share"] = vals["SQ_WAIT_INST_ANY"] / vals["SQ_WAVE_CYCLES"]
+    return out
+
+
+def valu_markdown(dbs, kernel, workload, pairs, flops_per_pair, kernel_ms, title, command):
+    vals = kernel_counters(dbs, kernel)
+    sm = valu_summary(vals, pairs, flops_per_pair, kernel_ms)
+    rec = {"workload": workload, "kernel": kernel, **sm}
+    lines = [f"# {title}", "", "Commands (from the repository root, on one MI355X):", "",
+             "```", command, "```", "",
+             f"Kernel `{kernel}`; {pairs:.6g} pairs per launch; kernel time {kernel_ms:.3f} ms "
+             f"(kernel-trace average of the same build on the same box).", "",
+             "| counter | value (last dispatch) | per pair (x64 lanes) |", "|---|---|---|"]
+    for c in sorted(vals):
+        pp = vals[c] * 64.0 / pairs if c.startswith("SQ_INSTS") else None
+        lines.append(f"| {c} | {vals[c]:.6g} | {'' if pp is None else f'{pp:.3f}'} |")
+    lines += ["", "| derived | value |", "|---|---|"]
+    for k, v in sm.items():
+        if k.endswith("_per_pair") or k in ("pairs", "kernel_ms"):
+            continue
+        lines.append(f"| {k} | {v:.4g} |")
+    lines += ["", "```json", json.dumps(rec), "```"]
+    return "\n".join(lines) + "\n"
+
+
+def load_record(workload: str, kernel: str, field: str):
+    """``field`` of the JSON record under profiles/ for (workload, kernel), or None."""
+    if not PROFILES.is_dir():
+        return None
+    for md in sorted(PROFILES.glob("*.md")):
+        for block in re.findall(r"```json\n(.*?)\n```", md.read_text(), flags=re.S):
+            try:
+                rec = json.loads(block)
+            except ValueError:
+                continue
+            if rec.get("workload") == workload and rec.get("kernel") == kernel and field in rec:
+                return rec[field]
+    return None
+
+
 def load_traffic(workload: str, kernel: str):
     """traffic_bytes recorded under profiles/ for (workload, kernel), or None."""
     if not PROFILES.is_dir():
@@ -133,7 +231,20 @@ def main(argv=None):
     t.add_argument("--workload", required=True)
     t.add_argument("--title", required=True)
     t.add_argument("--command", required=True)
+    v = sub.add_parser("valu")
+    v.add_argument("dbs", nargs="+")
+    v.add_argument("--kernel", required=True)
+    v.add_argument("--workload", required=True)
+    v.add_argument("--pairs", type=float, required=True)
+    v.add_argument("--flops-per-pair", type=float, required=True)
+    v.add_argument("--kernel-ms", type=float, required=True)
+    v.add_argument("--title", required=True)
+    v.add_argument("--command", required=True)
     a = ap.parse_args(argv)
+    if a.cmd == "valu":
+        print(valu_markdown(a.dbs, a.kernel, a.workload, a.pairs, a.flops_per_pair, a.kernel_ms,
+                            a.title, a.command), end="")
+        return
     if a.cmd == "stats":
         print(stats_markdown(a.db, a.title, a.command), end="")
     else:

@@ -46,7 +46,9 @@ class ParticleSystem:
     """Manages particles on the HIP device and advances them in time.
 
     Arguments are those of vpmModule.py:49-61, plus the keyword-only
-    ``precision``, ``device``, ``verbose`` and ``write_backups``.
+    ``precision``, ``device``, ``verbose``, ``write_backups`` and
+    ``check_finite`` (scan the state after every update_state and raise
+    NonFiniteError on NaN/Inf, see check_state).
     """
 
     def __init__(self, time_step_size: float = 1.0, time_integration_method: str = "RK2",
@@ -55,7 +57,7 @@ class ParticleSystem:
                  flow_time: float = 0.0, time_step: int = 0, backup_frequency: int = 1,
                  backup_filename: str = "particle_data", relax_strength_solution: bool = False,
                  *, precision: str = "plain", device=None, verbose: bool = False,
-                 write_backups: bool = True):
+                 write_backups: bool = True, check_finite: bool = False):
         if monitor_variables is None:                                 # :65-71
             monitor_variables = ["Circulation", "Linear Impulse", "Angular Impulse",
                                  "Kinetic Energy", "Enstrophy", "Helicity"]
@@ -85,6 +87,7 @@ class ParticleSystem:
         self.precision = precision
         self.verbose = verbose
         self.write_backups = write_backups
+        self.check_finite = check_finite
         self.variant = "cuda" if processing_unit == "GPU" else "cpu"
         self.physics = ParticlesPhysicsHip                           # :141-154
         # C_nu, enstrophy_ref, has_ref: device state of the LES filter
@@ -528,6 +531,16 @@ class ParticleSystem:
         self.time_step += 1
         self._log(f"\n>>> Flow time: {self.flow_time:0.3E} s, Time-step: {self.time_step:d}\n")
         self._pos = new_positions
+        if self.check_finite:
+            self.check_state(f"state after time-step {self.time_step}")
+
+    def check_state(self, where="particle state"):
+        """Raises NonFiniteError (status VPM_ERR_NONFINITE) if any position or
+        strength is NaN/Inf, a radius is not > 0 or a viscosity is negative
+        (one device scan and a 4-byte read back).  No reference counterpart:
+        the reference carries NaN on silently."""
+        if self.n:
+            P.check_particles(self._pos, self._str, self._sigma, self._nu, where=where)
 
     def _axpy(self, x, y, c):
         out = torch.empty_like(x)

@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from .build_cpu import LIB, build
+from .build_cpu import LIB, build, runtime_lib
 
 _lib = None
 _D = C.POINTER(C.c_double)
@@ -23,12 +23,21 @@ def load():
     if _lib is None:
         if not LIB.exists():
             build()
-        lib = C.CDLL(str(LIB))
+        path, _ = runtime_lib()
+        lib = C.CDLL(str(path))
         for name in ("vpm_cpu_selfinduced_velocity", "vpm_cpu_strength_gradients"):
             getattr(lib, name).argtypes = [C.c_int64, _D, _D, _D, C.c_int64, _L, _D]
             getattr(lib, name).restype = None
         lib.vpm_cpu_strength_gradients_pse.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int64, _L, _D]
         lib.vpm_cpu_strength_gradients_pse.restype = None
+        for name in ("vpm_cpu_cuda_selfinduced_velocity", "vpm_cpu_cuda_vorticity"):
+            getattr(lib, name).argtypes = [C.c_int64, _D, _D, _D, C.c_int64, _L, _D]
+            getattr(lib, name).restype = None
+        lib.vpm_cpu_cuda_strength_gradients_pse.argtypes = [C.c_int64, _D, _D, _D, _D, C.c_int64,
+                                                            _L, _D]
+        lib.vpm_cpu_cuda_strength_gradients_pse.restype = None
+        lib.vpm_cpu_cuda_enstrophy_rows.argtypes = [C.c_int64, _D, _D, _D, _D]
+        lib.vpm_cpu_cuda_enstrophy_rows.restype = None
         lib.vpm_cpu_max_threads.restype = C.c_int
         lib.vpm_cpu_set_threads.argtypes = [C.c_int]
         lib.vpm_cpu_set_threads.restype = None
@@ -70,6 +79,75 @@ def strength_gradients_pse(x, a, s, nu, tidx=None):
     load().vpm_cpu_strength_gradients_pse(len(x), _p(x), _p(a), _p(s), _p(nu), len(tidx),
                                           _p(tidx, _L), _p(out))
     return out
+
+
+# ---- the GPU plugin's forms (ParticlesPhysicsCuda.py) -------------------
+def cuda_selfinduced_velocity(x, a, s, tidx=None):
+    """ParticlesPhysicsCuda.py:133-191 (sigma_ij mean, j != i)."""
+    x, a, s, tidx, out = _prep(x, a, s, tidx)
+    load().vpm_cpu_cuda_selfinduced_velocity(len(x), _p(x), _p(a), _p(s), len(tidx),
+                                             _p(tidx, _L), _p(out))
+    return out
+
+
+def cuda_vorticity(x, a, s, tidx=None):
+    """ParticlesPhysicsCuda.py:291-346 at the particles' own positions."""
+    x, a, s, tidx, out = _prep(x, a, s, tidx)
+    load().vpm_cpu_cuda_vorticity(len(x), _p(x), _p(a), _p(s), len(tidx), _p(tidx, _L), _p(out))
+    return out
+
+
+def cuda_strength_gradients_pse(x, a, s, nu, tidx=None):
+    """ParticlesPhysicsCuda.py:478-530 (nu_i, A = rho^2 + 1)."""
+    x, a, s, tidx, out = _prep(x, a, s, tidx)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    load().vpm_cpu_cuda_strength_gradients_pse(len(x), _p(x), _p(a), _p(s), _p(nu), len(tidx),
+                                               _p(tidx, _L), _p(out))
+    return out
+
+
+def cuda_total_enstrophy(x, a, s):
+    """ParticlesPhysicsCuda.py:756-777,867-920: per-target rows summed in order."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    rows = np.empty(len(x))
+    load().vpm_cpu_cuda_enstrophy_rows(len(x), _p(x), _p(a), _p(s), _p(rows))
+    return float(np.sum(rows)) / (8.0 * np.pi) / (4.0 * np.pi)
+
+
+class CudaSums:
+    """The pair sums a ReferenceStepper needs (GPU-plugin forms), from this
+    library instead of the NumPy oracle: full-resolution steps in seconds."""
+
+    @staticmethod
+    def selfinduced_velocity(x, a, s, variant="cuda"):
+        assert variant == "cuda"
+        return cuda_selfinduced_velocity(x, a, s)
+
+    @staticmethod
+    def vorticity_field_at(y, x, a, s, variant="cuda"):
+        assert variant == "cuda" and y is x
+        return cuda_vorticity(x, a, s)
+
+    @staticmethod
+    def total_enstrophy(x, a, s, variant="cuda"):
+        assert variant == "cuda"
+        return cuda_total_enstrophy(x, a, s)
+
+    @staticmethod
+    def strength_gradients(x, a, s):
+        return strength_gradients(x, a, s)
+
+    @staticmethod
+    def strength_gradients_pse(x, a, s, nu, variant="cuda"):
+        assert variant == "cuda"
+        return cuda_strength_gradients_pse(x, a, s, nu)
+
+
+def isa() -> str:
+    """ISA level of the library loaded on this host (x86-64-v3 or -v4)."""
+    return runtime_lib()[1]
 
 
 def threads() -> int:

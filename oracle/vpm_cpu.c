@@ -9,6 +9,8 @@
  * fast-math pow simplification gives Numba's code; it also keeps GCC's
  * vectorised pow out (GCC 11 -O3 -ffast-math -march=x86-64-v3 computes wrong
  * values with it).
+ * The vpm_cpu_cuda_* functions restate the GPU plugin's forms instead
+ * (ParticlesPhysicsCuda.py), for full-resolution solver-step checks.
  * Arrays are the reference's AoS (N,3) float64 rows.  Each function
  * evaluates the targets listed in tidx[0..m) against all n sources (the
  * full evaluation is tidx = 0..n-1), so a strided target sample can be timed.
@@ -120,6 +122,118 @@ void vpm_cpu_strength_gradients_pse(int64_t n, const double* x, const double* a,
     out[3 * k] = dx * ONE_OVER_FOUR_PI;
     out[3 * k + 1] = dy * ONE_OVER_FOUR_PI;
     out[3 * k + 2] = dz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ---- GPU-plugin arithmetic (ParticlesPhysicsCuda.py), FP64 -------------
+ * The same sums in the GPU plugin's forms, for checking full-resolution
+ * solver steps in seconds where the NumPy oracle would take minutes.  They
+ * are pinned against the NumPy oracle (variant "cuda") at small N. */
+
+/* ParticlesPhysicsCuda.py:133-191: sigma_ij = (sigma_i + sigma_j)/2, j != i. */
+void vpm_cpu_cuda_selfinduced_velocity(int64_t n, const double* x, const double* a,
+                                       const double* s, int64_t m, const int64_t* tidx,
+                                       double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double si = s[i];
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      const double sg = (si + s[j]) / 2.0;
+      const double sg2 = sg * sg;
+      const double rho2 = r2 / sg2;
+      const double q = (rho2 + 2.5) / (pow25(rho2 + 1.0) * sg * sg2);
+      const double ax = a[3 * j], ay = a[3 * j + 1], az = a[3 * j + 2];
+      ux -= q * (ry * az - rz * ay);
+      uy -= q * (rz * ax - rx * az);
+      uz -= q * (rx * ay - ry * ax);
+    }
+    out[3 * k] = ux * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = uy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = uz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ParticlesPhysicsCuda.py:291-346 (simplified form): sigma_j, r^2 > 1e-8.
+ * Targets are the sources' own positions (tidx), as vpmModule.py calls it. */
+void vpm_cpu_cuda_vorticity(int64_t n, const double* x, const double* a, const double* s,
+                            int64_t m, const int64_t* tidx, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      if (!(r2 > EPSILON)) continue;
+      const double sj = s[j];
+      const double rho2 = r2 / (sj * sj);
+      const double xi = 7.5 / (pow35(rho2 + 1.0) * (sj * sj * sj));
+      wx += xi * a[3 * j];
+      wy += xi * a[3 * j + 1];
+      wz += xi * a[3 * j + 2];
+    }
+    out[3 * k] = wx * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = wy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = wz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ParticlesPhysicsCuda.py:478-530: nu_i, A = rho^2 + 1, j != i. */
+void vpm_cpu_cuda_strength_gradients_pse(int64_t n, const double* x, const double* a,
+                                         const double* s, const double* nu, int64_t m,
+                                         const int64_t* tidx, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < m; ++k) {
+    const int64_t i = tidx[k];
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double ix = a[3 * i], iy = a[3 * i + 1], iz = a[3 * i + 2];
+    const double vis = nu[i];
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double sg = (s[i] + s[j]) * 0.5;
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      const double sg2 = sg * sg, sg3 = sg2 * sg;
+      const double A = r2 / sg2 + 1.0;
+      const double w = 105.0 * vis / (sg2 * sg3 * pow45(A)) * sg3;
+      dx += w * (a[3 * j] - ix);
+      dy += w * (a[3 * j + 1] - iy);
+      dz += w * (a[3 * j + 2] - iz);
+    }
+    out[3 * k] = dx * ONE_OVER_FOUR_PI;
+    out[3 * k + 1] = dy * ONE_OVER_FOUR_PI;
+    out[3 * k + 2] = dz * ONE_OVER_FOUR_PI;
+  }
+}
+
+/* ParticlesPhysicsCuda.py:867-920 + :777: i = j included, 1/(8pi) 1/(4pi).
+ * Per-target partial sums in out[0..n), added in index order by the caller
+ * side (deterministic for any thread count). */
+void vpm_cpu_cuda_enstrophy_rows(int64_t n, const double* x, const double* a, const double* s,
+                                 double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const double tx = x[3 * i], ty = x[3 * i + 1], tz = x[3 * i + 2];
+    const double ix = a[3 * i], iy = a[3 * i + 1], iz = a[3 * i + 2];
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      const double rx = tx - x[3 * j], ry = ty - x[3 * j + 1], rz = tz - x[3 * j + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      const double sg = (s[i] + s[j]) * 0.5;
+      const double sg2 = sg * sg;
+      const double d = r2 + sg2;
+      const double t1 = (r2 + 3.2 * sg2) / (d * sqrt(d));
+      acc += t1 * (ix * a[3 * j] + iy * a[3 * j + 1] + iz * a[3 * j + 2]);
+    }
+    out[i] = acc;
   }
 }
 

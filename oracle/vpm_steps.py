@@ -25,7 +25,7 @@ class ReferenceStepper:
 
     def __init__(self, x, alpha, sigma, nu, *, dt, method="RK4", flow_model="DNS",
                  viscous_scheme="PSE", variant="cuda", relax=False, nu_t=None,
-                 time_step=0, flow_time=0.0):
+                 time_step=0, flow_time=0.0, sums=None):
         self.x = np.array(x, dtype=np.float64)
         self.alpha = np.array(alpha, dtype=np.float64)
         self.sigma = np.array(sigma, dtype=np.float64)
@@ -45,13 +45,16 @@ class ReferenceStepper:
         self.prev = None                              # created lazily, :741-746
         self.prev2 = None
         self.prev3 = None
+        # pair sums: the NumPy oracle, or an object with the same five
+        # functions (oracle.cpu_baseline.CudaSums: the C port, GPU-plugin forms)
+        self.sums = O if sums is None else sums
 
     @property
     def nu_eff(self):
         return self.nu + self.nu_t                    # Particle.py:61,104-105
 
     def _vel(self, x):
-        return O.selfinduced_velocity(x, self.alpha, self.sigma, self.variant)
+        return self.sums.selfinduced_velocity(x, self.alpha, self.sigma, self.variant)
 
     # vpmModule.py:699-710
     def update_velocities(self):
@@ -84,8 +87,8 @@ class ReferenceStepper:
             return
         x, s, radii = self.x, self.alpha.copy(), self.sigma
         if self.flow_model == "LES":                          # :731-737
-            vort = O.vorticity_field_at(x, x, s, radii, self.variant)
-            ens = O.total_enstrophy(x, s, radii, self.variant)
+            vort = self.sums.vorticity_field_at(x, x, s, radii, self.variant)
+            ens = self.sums.total_enstrophy(x, s, radii, self.variant)
             self.cnu, self.enstrophy_ref = O.les_filter_constant(ens, self.cnu, self.enstrophy_ref)
             nut, write = O.les_particle_viscosity(vort, radii, self.cnu)
             if write:
@@ -97,19 +100,19 @@ class ReferenceStepper:
         if self.prev3 is None:
             self.prev3 = np.zeros_like(s)
         if self.flow_model in ("LES", "DNS"):                 # :749-791
-            rhs = O.strength_gradients(x, s, radii)
+            rhs = self.sums.strength_gradients(x, s, radii)
             s = self._bdf(s, rhs)
             self._push_history(s)
             self.alpha = s.copy()
         if self.viscous_scheme == "PSE":                      # :797-845
-            rhs = O.strength_gradients_pse(x, s, radii, self.nu_eff, self.variant)
+            rhs = self.sums.strength_gradients_pse(x, s, radii, self.nu_eff, self.variant)
             s = self._bdf(s, rhs)
             self._push_history(s)
             self.alpha = s.copy()
         elif self.viscous_scheme == "CoreSpreading":          # :851-862
             self.sigma = O.core_spreading(radii, self.nu_eff, self.dt)
         if self.relax:                                        # :867-883
-            vort = O.vorticity_field_at(self.x, self.x, self.alpha, self.sigma, self.variant)
+            vort = self.sums.vorticity_field_at(self.x, self.x, self.alpha, self.sigma, self.variant)
             vn = np.sqrt(np.sum(vort * vort, axis=1))
             sn = np.sqrt(np.sum(self.alpha * self.alpha, axis=1))
             self.alpha = self.alpha * 0.95 + 0.05 * (vort / vn[:, None]) * sn[:, None]

@@ -173,3 +173,47 @@ def test_batch_rk4_equals_single_set_rk4(gpu):
                                          sigma_mode="uniform", pse=True)
         assert rel(bs.x[:, sl], xn) <= 1e-13, k
         assert rel(bs.a[:, sl], an) <= 1e-11, k
+
+
+def test_batch_autotune_caches_and_is_used(gpu, tmp_path):
+    """tune_batch times (tile shape x persistent grid) candidates on the sets,
+    checks each against the FP64 batched pass, caches the fastest accurate
+    one; BatchedSystem(tile_config="auto") then launches with it and stays
+    within the bound."""
+    from openonda_amd import autotune as AT
+
+    sets = small_sets(3)
+    cache = tmp_path / "tune.json"
+    best, report = AT.tune_batch(sets, configs=[0, 1, 4], grids=[0, 64], reps=1, path=cache)
+    assert len(report) == 6 and all(r["ms"] > 0 for r in report)
+    assert best is not None and best["rel_l2"] <= 1e-6
+    bs = B.BatchedSystem(sets, dt=0.01, tile_config="auto", autotune_cache=cache)
+    assert bs.autotuned == (best["config"], best["grid"])
+    assert (bs.tile_config, bs.grid) == (best["config"], best["grid"])
+    ref = B.BatchedSystem(sets, dt=0.01, precision="f64")
+    for b in (bs, ref):
+        b.evaluate(b.x, b.a, b.stage_mask, True, b.stream.cuda_stream)
+        b.synchronize()
+    for k in range(len(sets)):
+        sl = bs.set_slice(k)
+        if bs.sizes[k] < 2:
+            continue
+        for got, want in ((bs.vel, ref.vel), (bs.str_, ref.str_), (bs.pse_, ref.pse_)):
+            assert rel(got[:, sl], want[:, sl]) <= 1e-6, k
+
+
+def test_batch_periodic_resort_in_replay(gpu):
+    """resort_every=K: every K-th step (eager or replayed) re-sorts; the
+    replayed run equals an eager run with the same re-sort schedule."""
+    sets = small_sets(4)
+    a = B.BatchedSystem(sets, dt=0.005, resort_every=2)
+    b = B.BatchedSystem(sets, dt=0.005, resort_every=2)
+    for _ in range(5):
+        a.step()
+    b.step()
+    b.capture()
+    b.replay(4)
+    a.synchronize()
+    b.synchronize()
+    assert a.steps_done == b.steps_done == 5
+    assert torch.equal(a.x, b.x) and torch.equal(a.a, b.a)

@@ -19,8 +19,9 @@ output.
 4. ring N = 131,072: FP32c velocity + stretching + PSE against the FP64 port,
    probe velocity against the NumPy oracle;
 5. 96 uneven sets in one persistent launch, before and after graph-replayed
-   RK4 steps with the LES filter (no re-sort in between), every set against
-   the FP64 port and the oracle.
+   RK4 steps with the LES filter (no re-sort in between): FP32c against the
+   FP64 batched pass on every particle of every set, the FP64 pass against
+   the FP64 port and the oracle on strided targets.
 """
 import numpy as np
 import pytest
@@ -47,7 +48,13 @@ def rel(a, b):
 
 
 def strided(n, m):
-    return np.arange(0, n, max(1, n // m), dtype=np.int64)[:m]
+    """m targets over [0, n) with an odd stride: the ring layouts hold 64
+    cross-section points per station, and an even stride would sample the
+    same few of them (a stride of 256 picks only core centres, whose
+    stretching vanishes by symmetry)."""
+    step = max(1, n // m)
+    step -= 1 - step % 2 if step > 1 else 0
+    return np.arange(0, n, step, dtype=np.int64)[:m]
 
 
 def dev(x, a, s, nu):
@@ -197,41 +204,66 @@ def test_config4_ring_forward_and_probes(gpu):
 
 
 # ------------------------------------------------------------------ config 5
-def _batch_vs_port(bs, sets, m, tag):
-    """Every set's velocity and stretching against the FP64 port, its PSE
-    (GPU-plugin form, nu_i = the set's nu + nu_t) against the NumPy oracle, on
-    m strided targets per set."""
+# FP64 against FP64 in another summation order: on the thin rings of the
+# batch the stretching is a near-cancelling sum (alpha_i x G with G nearly
+# parallel to alpha_i; the terms exceed the result by 1e4-1e5), so two exact
+# orders differ by up to ~1e-11 of it; velocity and PSE stay at 1e-12.
+FP64_TOL = {"vel": 1e-12, "stretch": 1e-10, "pse": 1e-12}
+
+
+def _batch_check(bs, ref, sets, m, tag):
+    """Every set: the FP32c outputs against the FP64 batched pass on all its
+    particles (rel-L2 <= 1e-6), and the FP64 batched pass against the FP64
+    port (velocity, stretching) and the NumPy oracle (PSE, GPU-plugin form
+    with nu_i = the set's nu + nu_t) on m strided targets (FP64_TOL).  A
+    strided sample of a ring set is too small to estimate an FP32 rel-L2 of
+    its near-cancelling stretching; the FP64 pass carries the independent
+    check instead."""
     worst = 0.0
     for k, (_, _, sg, _) in enumerate(sets):
         sl = bs.set_slice(k)
-        x = P.from_soa(bs.x[:, sl])
-        a = P.from_soa(bs.a[:, sl])
-        nu = bs.nu_eff[sl].cpu().numpy()
-        idx = strided(len(x), m)
-        ref = port(x, a, sg, nu, idx, ("vel", "stretch", "pse"))
-        for key, t in (("vel", bs.vel), ("stretch", bs.str_), ("pse", bs.pse_)):
-            e = rel(rows(t[:, sl], idx), ref[key])
+        for key, t32, t64 in (("vel", bs.vel, ref.vel), ("stretch", bs.str_, ref.str_),
+                              ("pse", bs.pse_, ref.pse_)):
+            nr = float(t64[:, sl].norm())
+            if nr == 0.0:
+                continue
+            e = float((t32[:, sl] - t64[:, sl]).norm()) / nr
             worst = max(worst, e)
-            assert e <= F32C_TOL, (tag, k, len(x), key, e)
+            assert e <= F32C_TOL, (tag, k, int(bs.sizes[k]), key, e)
+        x = P.from_soa(ref.x[:, sl])
+        a = P.from_soa(ref.a[:, sl])
+        nu = ref.nu_eff[sl].cpu().numpy()
+        idx = strided(len(x), m)
+        want = port(x, a, sg, nu, idx, ("vel", "stretch", "pse"))
+        for key, t in (("vel", ref.vel), ("stretch", ref.str_), ("pse", ref.pse_)):
+            assert rel(rows(t[:, sl], idx), want[key]) <= FP64_TOL[key], (tag, k, key)
     return worst
 
 
-def test_config5_batch_against_fp64_port(gpu):
+def test_config5_batch_against_fp64(gpu):
     sets = FM.batch_instances()
-    print(f"[config5] {len(sets)} sets, one persistent launch vs FP64 port", flush=True)
-    bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True)
-    s = bs.stream.cuda_stream
+    print(f"[config5] {len(sets)} sets, one persistent launch, FP32c vs FP64 vs port", flush=True)
     mask = 1 | 2 | 4
+    bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True)
+    ref = B.BatchedSystem(sets, dt=0.005, precision="f64", pse=True, les=True)
+    s = bs.stream.cuda_stream
     bs.evaluate(bs.x, bs.a, mask, True, s)
+    ref.evaluate(ref.x, ref.a, mask, True, ref.stream.cuda_stream)
     bs.synchronize()
-    w0 = _batch_vs_port(bs, sets, 32, "step0")
+    ref.synchronize()
+    w0 = _batch_check(bs, ref, sets, 32, "step0")
     # graph-replayed RK4 steps with the order sorted at the first step; the
-    # evaluation below reuses that order (no re-sort), as the replays do
+    # FP64 pass then evaluates the advanced FP32c state (positions, strengths,
+    # nu_eff) and both reuse their sorted order (no re-sort), as replays do
     bs.step()
     bs.capture()
     bs.replay(5)
     bs.synchronize()
+    for dst, src in ((ref.x, bs.x), (ref.a, bs.a), (ref.nu_eff, bs.nu_eff)):
+        dst.copy_(src)
     bs.evaluate(bs.x, bs.a, mask, False, s)
+    ref.evaluate(ref.x, ref.a, mask, True, ref.stream.cuda_stream)
     bs.synchronize()
-    w1 = _batch_vs_port(bs, sets, 32, "after 6 steps")
+    ref.synchronize()
+    w1 = _batch_check(bs, ref, sets, 32, "after 6 steps")
     print(f"[config5] worst rel-L2 step 0 {w0:.2e}, after 6 steps {w1:.2e}", flush=True)

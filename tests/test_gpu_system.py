@@ -14,6 +14,7 @@ import torch
 from oracle import vpm_oracle as O
 from oracle.vpm_steps import ReferenceStepper
 from openonda_amd import flow_models as FM
+from openonda_amd import physics as P
 from openonda_amd.Particle import Particle
 from openonda_amd.vpmModule import ParticleSystem
 
@@ -144,12 +145,14 @@ def test_impulses_match_oracle_on_asymmetric_sets(gpu, case):
     assert np.allclose(ps.get_total_strength(), a.sum(0), rtol=1e-12, atol=1e-14 * mag)
 
 
-@pytest.mark.parametrize("threshold", [1e-7, 1e-30])
+@pytest.mark.parametrize("threshold", [2.1e-4, 1e-30])
 def test_weak_particle_removal_mid_run_keeps_the_strength_history(gpu, threshold):
     """remove_weak_particles between steps 3 and 4 of a multistep strength
     update (vpmModule.py:970-1037): the scheme continues on the remaining
     particles' history.  threshold 1e-30 removes nothing (the reference then
-    continues unchanged); 1e-7 removes the Gaussian core's outermost points."""
+    continues unchanged); 2.1e-4 removes the 80 weakest points of the core's
+    outer ring (relative magnitudes after 3 steps run 1.9e-4 ... 2.09e-4,
+    then 2.14e-4: a gap wide against any rounding difference)."""
     print(f"[system] removal mid-run threshold={threshold}", flush=True)
     x, a, s, nu = FM.torus_ring(8, nu=1e-3)
     dt = 0.02
@@ -286,3 +289,42 @@ def test_checkpoint_resume_is_bit_identical(gpu, tmp_path, flow, visc):
                 "get_particle_velocities", "get_particle_viscosities_t", "get_particle_group_id"):
         assert np.array_equal(getattr(q, get)(), getattr(ps, get)()), get
     assert q.Cnu == ps.Cnu and q.time_step == ps.time_step
+
+
+def test_device_scan_flags_non_finite_state(gpu):
+    """vpm_scan_particles through the C ABI: a clean set gives 0; a NaN
+    position, an Inf strength, a zero radius and a negative viscosity each
+    set their VPM_BAD_* bit; ParticleSystem(check_finite=True) raises
+    NonFiniteError after the step that produced a NaN."""
+    from openonda_amd import _lib as L
+
+    x, a, s, nu = FM.torus_ring(8, nu=1e-3)
+    pos, al = P.to_soa(x), P.to_soa(a)
+    sig, vis = torch.from_numpy(s).cuda(), torch.from_numpy(nu).cuda()
+    assert P.scan_particles(pos, al, sig, vis) == 0
+    cases = [(pos, 1, 7, float("nan"), L.BAD_POSITION), (al, 2, 100, float("inf"), L.BAD_STRENGTH),
+             (sig, None, 300, 0.0, L.BAD_RADIUS), (vis, None, 511, -1e-6, L.BAD_VISCOSITY)]
+    for t, c, i, v, bit in cases:
+        t2 = t.clone()
+        if c is None:
+            t2[i] = v
+        else:
+            t2[c, i] = v
+        args = [pos, al, sig, vis]
+        args[next(q for q, u in enumerate(args) if u is t)] = t2
+        assert P.scan_particles(*args) == bit
+        with pytest.raises(L.NonFiniteError) as e:
+            P.check_particles(*args)
+        assert e.value.code == L.VPM_ERR_NONFINITE and e.value.bits == bit
+    ps = ParticleSystem(time_step_size=0.01, time_integration_method="RK2", processing_unit="GPU",
+                        flow_model="DNS", viscous_scheme="CoreSpreading", write_backups=False,
+                        check_finite=True)
+    ps.add_particle_field(x, np.zeros_like(x), a, s, nu)
+    ps.update_velocities()
+    ps.update_strengths()
+    ps.update_state()                       # clean: no error
+    ps._str[1, 5] = float("nan")            # a NaN strength reaches the next step
+    ps.update_velocities()
+    ps.update_strengths()
+    with pytest.raises(L.NonFiniteError, match="time-step 2"):
+        ps.update_state()

@@ -147,3 +147,41 @@ def test_drop_in_precision_switch_checks():
             H._fused()
     finally:
         H.PRECISION, H.VARIANT = "plain", "cuda"
+
+
+@pytest.mark.parametrize("which, bit", [("x", 1), ("a", 2), ("s", 4), ("nu", 8), ("y", 1)])
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_drop_in_rejects_non_finite_inputs_before_any_gpu_call(which, bit, bad):
+    """NaN/Inf inputs (and radii <= 0, viscosities < 0) raise NonFiniteError
+    with status VPM_ERR_NONFINITE and the VPM_BAD_* bit of the array, on the
+    host, before the library or a device is touched."""
+    from openonda_amd import ParticlesPhysicsHip as H
+    from openonda_amd import _lib as L
+
+    x, a, s, nu = np.zeros((4, 3)), np.ones((4, 3)), np.full(4, 0.1), np.full(4, 1e-3)
+    y = np.zeros((2, 3))
+    arrs = dict(x=x, a=a, s=s, nu=nu, y=y)
+    arrs[which] = arrs[which].copy()
+    arrs[which].flat[1] = bad
+    if which == "y":
+        call = lambda: H.get_induced_velocity_at(arrs["y"], arrs["x"], arrs["a"], arrs["s"])
+    elif which == "nu":
+        call = lambda: H.get_strength_gradients_PSE(arrs["x"], arrs["a"], arrs["s"], arrs["nu"])
+    else:
+        call = lambda: H.get_selfinduced_velocity(arrs["x"], arrs["a"], arrs["s"])
+    with pytest.raises(L.NonFiniteError) as e:
+        call()
+    assert e.value.code == L.VPM_ERR_NONFINITE == -6
+    assert e.value.bits == bit
+
+
+def test_drop_in_rejects_non_positive_radii_and_negative_viscosity():
+    from openonda_amd import ParticlesPhysicsHip as H
+    from openonda_amd import _lib as L
+
+    x, a = np.zeros((3, 3)), np.ones((3, 3))
+    with pytest.raises(L.NonFiniteError, match="radii"):
+        H.get_strength_gradients(x, a, np.array([0.1, 0.0, 0.1]))
+    with pytest.raises(L.NonFiniteError, match="viscosities"):
+        H.get_strength_gradients_PSE(x, a, np.full(3, 0.1), np.array([1e-3, -1e-9, 0.0]))
+    assert issubclass(L.NonFiniteError, ValueError)

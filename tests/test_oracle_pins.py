@@ -361,6 +361,37 @@ def test_c_port_matches_numpy_oracle():
     assert rel(CB.strength_gradients(x, a, s, tidx), O.strength_gradients(x, a, s)[tidx]) < 1e-13
 
 
+def test_c_port_gpu_plugin_forms_match_numpy_oracle():
+    """The C port's GPU-plugin sums (used for full-resolution step checks)
+    against the NumPy oracle's "cuda" variant, with non-uniform radii and
+    viscosities and a coincident pair; and a ReferenceStepper step driven by
+    them against one driven by the NumPy oracle (LES, core spreading, RK2)."""
+    from oracle import cpu_baseline as CB
+    from oracle.vpm_steps import ReferenceStepper
+
+    x, a, s, nu = rand_set(500, 12)
+    x[9] = x[3]
+    rel = lambda p, q: np.linalg.norm(p - q) / np.linalg.norm(q)
+    assert rel(CB.cuda_selfinduced_velocity(x, a, s), O.selfinduced_velocity(x, a, s, "cuda")) < 1e-13
+    assert rel(CB.cuda_vorticity(x, a, s), O.vorticity_field_at(x, x, a, s, "cuda")) < 1e-13
+    assert rel(CB.cuda_strength_gradients_pse(x, a, s, nu),
+               O.strength_gradients_pse(x, a, s, nu, "cuda")) < 1e-13
+    assert CB.cuda_total_enstrophy(x, a, s) == pytest.approx(O.total_enstrophy(x, a, s, "cuda"),
+                                                             rel=1e-13)
+    tidx = np.array([9, 0, 499])
+    assert rel(CB.cuda_selfinduced_velocity(x, a, s, tidx),
+               O.selfinduced_velocity(x, a, s, "cuda")[tidx]) < 1e-13
+    kw = dict(dt=1e-3, method="RK2", flow_model="LES", viscous_scheme="CoreSpreading")
+    r0 = ReferenceStepper(x, a, s, nu, **kw)
+    r1 = ReferenceStepper(x, a, s, nu, sums=CB.CudaSums, **kw)
+    for _ in range(2):
+        r0.step()
+        r1.step()
+    assert r1.cnu == pytest.approx(r0.cnu, rel=1e-13)
+    for k in ("x", "alpha", "sigma", "nu_t"):
+        assert rel(getattr(r1, k), getattr(r0, k)) < 1e-12, k
+
+
 def test_row_subsets_equal_the_full_evaluation():
     """The rows= option of the self-evaluations returns exactly those rows."""
     x, a, s, nu = rand_set(300, 21)
