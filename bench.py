@@ -365,13 +365,10 @@ def main(argv=None):
     if not args.skip_batch:
         from openonda_amd import batch as B
 
-        from openonda_amd import autotune as AT
-
         sets = FM.batch_instances()
-        stage = L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE | L.OUT_VORT | L.OUT_ENSTROPHY
-        if AT.best_batch_config([len(q[0]) for q in sets], stage) is None:
-            log("[bench] batch: tuning the persistent launch (tile shape x grid; untimed)")
-            AT.tune_batch(sets, reps=1)
+        # the launch the batch autotuner cached for this workload, else the
+        # library's default shape (the sweep itself is limited to smaller
+        # batches: autotune.TUNE_BATCH_MAX_PARTICLES)
         bs = B.BatchedSystem(sets, dt=0.005, precision="f32c", pse=True, les=True,
                              tile_config="auto")
         log(f"[bench] batch: {bs.nsets} sets, N_total={bs.n}, {bs.n_items} work items, "

@@ -152,6 +152,15 @@ def device_tune(pos, alpha, sigma, nu=None, outputs=("vel", "stretch"), sigma_mo
 
 
 # ---------------------------------------------------------------- batch
+# The sweep over every (tile shape x grid) candidate is limited to batches
+# of this many particles: run over the 96-set workload (3.3e6 particles) it
+# faulted the GPU twice.  The cause found in the code -- the kernels without
+# far records wrote per-wave target boxes past their one-row LDS array,
+# where other waves of the batch kernel could read the shared work-item
+# index -- is fixed, but the full-size sweep has not been run since.
+TUNE_BATCH_MAX_PARTICLES = 1 << 18
+
+
 def _batch_key(sizes, mask, precision="f32c"):
     pairs = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
     bucket = int(round(math.log2(pairs))) if pairs > 1 else 0
@@ -169,7 +178,7 @@ def best_batch_config(sizes, mask, precision="f32c", path=None):
 
 
 def tune_batch(sets, pse=True, les=True, configs=None, grids=None, reps=2, max_rel=1e-6,
-               path=None, save=True):
+               path=None, save=True, log=None):
     """Times the batched persistent launch (one stage-1 evaluation: velocity,
     stretching and, as requested, PSE and the LES outputs, over every set) for
     each tile configuration x persistent grid size on these sets, checks every
@@ -178,10 +187,15 @@ def tune_batch(sets, pse=True, les=True, configs=None, grids=None, reps=2, max_r
     sets' output mask and total pair count.  ``grids``: persistent workgroup
     counts (0: as many as fit; default 0 and 1-3 per CU).  Returns (best,
     report); best is None (nothing cached, a warning) when no candidate meets
-    the bound."""
+    the bound.  ``log``: a callable that receives one line per candidate."""
     from . import batch as B
     from . import _lib as L
 
+    total = sum(len(q[0]) for q in sets)
+    if total > TUNE_BATCH_MAX_PARTICLES:
+        raise NotImplementedError(
+            f"tune_batch: {total} particles exceed {TUNE_BATCH_MAX_PARTICLES}; the sweep over "
+            "every launch shape is run on smaller batches only (see the module notes)")
     P.require_gpu()
     ref = B.BatchedSystem(sets, dt=0.0, precision="f64", pse=pse, les=les)
     mask = ref.stage_mask
@@ -223,6 +237,8 @@ def tune_batch(sets, pse=True, les=True, configs=None, grids=None, reps=2, max_r
                 worst = max(worst, float(((bs.enstrophy - refe).abs() /
                                           refe.abs().clamp_min(1e-300)).max()))
             report.append({"config": cfg, "grid": grid, "ms": ms, "rel_l2": worst})
+            if log is not None:
+                log(f"[tune_batch] config {cfg} grid {grid}: {ms:.2f} ms, rel-L2 {worst:.2e}")
     ok = [r for r in report if r["rel_l2"] <= max_rel]
     if not ok:
         warnings.warn(f"tune_batch: no launch reached rel-L2 <= {max_rel:g}; nothing cached",

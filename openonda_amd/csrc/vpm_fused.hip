@@ -676,8 +676,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       if (lane == 0) {
         red[c][wave] = lo;
         red[3 + c][wave] = hi;
-        wbox[wave][c] = 0.5 * (lo + hi);
-        wbox[wave][3 + c] = 0.5 * (hi - lo) * scale;
+        if constexpr (kFarS) {  // wbox has one row per wave only then
+          wbox[wave][c] = 0.5 * (lo + hi);
+          wbox[wave][3 + c] = 0.5 * (hi - lo) * scale;
+        }
       }
     }
     if (lane == 0) wave_coh[wave] = coh ? 1 : 0;
@@ -1583,7 +1585,9 @@ void fused_batch_kernel(BatchArgs b) {
       item_s = __hip_atomic_fetch_add(b.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int item = item_s;
-    if (item >= b.n_items) break;  // every workgroup ends once the list is drained
+    // every workgroup ends once the list is drained (or on a counter that is
+    // not a valid item: its workspace was overwritten)
+    if (item < 0 || item >= b.n_items) break;
     const int inst = b.item_inst[item];
     const int off = b.inst_off[inst];
     const int n = b.inst_off[inst + 1] - off;
