@@ -192,6 +192,21 @@ int vpm_forward_prepare(const vpm_particles_t* p, const vpm_forward_opts_t* opts
 int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
                     vpm_forward_out_t* out, void* workspace, size_t ws_bytes, void* stream);
 
+/* vpm_forward without a caller workspace and without any allocation: the
+ * sort, the records and the split partials live in a static device array of
+ * vpm_forward_noalloc_capacity() bytes (256 MiB) per GPU, whose address is
+ * looked up once per device.  Returns VPM_ERR_WORKSPACE when n needs more
+ * than that (vpm_forward_workspace_bytes without the split partials); the
+ * source-split partials get what is left, so small sets may split less than
+ * vpm_forward.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (different streams or threads) race.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+int vpm_forward_noalloc(const vpm_particles_t* p, const vpm_forward_opts_t* opts,
+                        vpm_forward_out_t* out, void* stream);
+size_t vpm_forward_noalloc_capacity(void);
+
 
 /* Tile configurations the fused F32C kernel is instantiated for (used by the
  * autotuner).  Writes block threads, targets per lane, source tile, sub-tile

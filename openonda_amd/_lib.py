@@ -127,6 +127,8 @@ SIGNATURES = {
     "vpm_forward": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),
     "vpm_forward_prepare": (C.c_int, [_P(Particles), _P(ForwardOpts), _VP, _SZ, _VP]),
     "vpm_forward_run": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),
+    "vpm_forward_noalloc": (C.c_int, [_P(Particles), _P(ForwardOpts), _P(ForwardOut), _VP]),
+    "vpm_forward_noalloc_capacity": (_SZ, []),
     "vpm_batch_workspace_bytes": (_SZ, [_I64, C.c_int, C.c_int]),
     "vpm_batch_prepare": (C.c_int, [_P(Particles), _P(BatchDesc), C.c_int, _VP, _SZ, _VP]),
     "vpm_batch_run": (C.c_int, [_P(Particles), _P(BatchDesc), _P(ForwardOpts), _P(ForwardOut), _VP, _SZ, _VP]),

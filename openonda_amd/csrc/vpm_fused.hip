@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "vpm_common.h"
@@ -2127,6 +2128,71 @@ int vpm_forward_run(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_f
   WsLayout L{};
   if ((rc = check_ws(p->n, o, workspace, ws_bytes, &L, "vpm_forward_run", false))) return rc;
   return forward_run_impl(p, o, out, static_cast<char*>(workspace), L, as_stream(stream));
+}
+
+// Workspace of vpm_forward_noalloc: a file-scope device array, one copy per
+// GPU (the code object's globals), held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is only a
+// stand-in for the symbol): its device address is looked up once per device
+// with hipGetSymbolAddress and kept in scratch_dev[].
+constexpr size_t kScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+void* scratch_dev[kMaxDevices];
+hipError_t scratch_err[kMaxDevices];
+std::once_flag scratch_once[kMaxDevices];
+
+// The current device's copy of `scratch`; the first call on a device looks
+// its address up (that device is current).
+int scratch_ptr(char** ptr) {
+  int dev = -1;
+  VPM_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(VPM_ERR_DEVICE, "vpm_forward_noalloc: device index out of range");
+  std::call_once(scratch_once[dev], [dev] {
+    scratch_err[dev] = hipGetSymbolAddress(&scratch_dev[dev], HIP_SYMBOL(scratch));
+  });
+  if (scratch_err[dev] != hipSuccess || !scratch_dev[dev]) {
+    set_error("vpm_forward_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+              hipGetErrorString(scratch_err[dev]));
+    return VPM_ERR_LAUNCH;
+  }
+  *ptr = static_cast<char*>(scratch_dev[dev]);
+  return VPM_OK;
+}
+
+}  // namespace
+
+size_t vpm_forward_noalloc_capacity(void) { return kScratchBytes; }
+
+int vpm_forward_noalloc(const vpm_particles_t* p, const vpm_forward_opts_t* o,
+                        vpm_forward_out_t* out, void* stream) {
+  clear_error();
+  int rc = check_forward(p, o, "vpm_forward_noalloc");
+  if (rc) return rc;
+  if ((rc = check_forward_out(p, o, out, "vpm_forward_noalloc"))) return rc;
+  // the layout without source-split partials must fit; the split partials
+  // then get what is left (fewer splits than vpm_forward may choose)
+  WsLayout L{};
+  if ((rc = layout(p->n, o, &L, true, 0))) return rc;
+  if (L.total > kScratchBytes) {
+    set_error("vpm_forward_noalloc: n = %lld needs %zu workspace bytes, more than the static "
+              "%zu (use vpm_forward with a caller workspace)",
+              (long long)p->n, L.total, kScratchBytes);
+    return VPM_ERR_WORKSPACE;
+  }
+  if ((rc = layout(p->n, o, &L, true, kScratchBytes - L.total))) return rc;
+  while (L.total > kScratchBytes && L.split_bytes > 0) {  // alignment slack
+    if ((rc = layout(p->n, o, &L, true, L.split_bytes - std::min<size_t>(L.split_bytes, 4096))))
+      return rc;
+  }
+  char* ws = nullptr;
+  if ((rc = scratch_ptr(&ws))) return rc;
+  hipStream_t st = as_stream(stream);
+  if ((rc = forward_prepare_impl(p, o, ws, L, st))) return rc;
+  return forward_run_impl(p, o, out, ws, L, st);
 }
 
 int vpm_forward(const vpm_particles_t* p, const vpm_forward_opts_t* o, vpm_forward_out_t* out,

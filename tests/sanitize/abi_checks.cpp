@@ -109,6 +109,12 @@ int main() {
   vpm_forward_out_t fo_nov{8, nullptr, out, nullptr, nullptr, nullptr};
   expect("forward_run vel missing", vpm_forward_run(&good, &o, &fo_nov, out, sizeof(out), nullptr), VPM_ERR_ARG);
   expect("forward_run null out", vpm_forward_run(&good, &o, nullptr, out, sizeof(out), nullptr), VPM_ERR_ARG);
+  expect("noalloc null out", vpm_forward_noalloc(&good, &o, nullptr, nullptr), VPM_ERR_ARG);
+  expect("noalloc ld<n", vpm_forward_noalloc(&ld, &o, &fo, nullptr), VPM_ERR_ARG);
+  if (vpm_forward_noalloc_capacity() != (size_t(256) << 20)) {
+    ++fails;
+    std::printf("FAIL noalloc capacity\n");
+  }
   int b = 0, r = 0, t = 0, s = 0, c = 0;
   expect("tile config info out of range", vpm_tile_config_info(-1, &b, &r, &t, &s, &c), VPM_ERR_ARG);
   expect("tile config info 0", vpm_tile_config_info(0, &b, &r, &t, &s, &c), VPM_OK);
