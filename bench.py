@@ -52,7 +52,9 @@ def log(*a):
 def pmc_probe(which, dev):
     """One launch of a secondary kernel (after one untimed warm-up launch),
     for counter runs: the ring of 131072 (velocity + stretching + PSE,
-    FP32-compensated), its adjoint, or one stage-1 launch of the 96-set batch."""
+    FP32-compensated), its adjoint, or the 96-set batch's launches of stage 1
+    (velocity + stretching + PSE + LES outputs) and of a later stage (velocity
+    + stretching + PSE)."""
     import torch
 
     from openonda_amd import autograd as AG
@@ -75,7 +77,11 @@ def pmc_probe(which, dev):
         from openonda_amd import batch as B
 
         bs = B.BatchedSystem(FM.batch_instances(), dt=0.005, precision="f32c", pse=True, les=True)
-        fn = lambda: bs.evaluate(bs.x, bs.a, bs.stage_mask, True, bs.stream.cuda_stream)
+
+        def fn():  # an RK4 step's stage 1 (with the LES outputs), then a later stage
+            bs.evaluate(bs.x, bs.a, bs.stage_mask, True, bs.stream.cuda_stream)
+            bs.evaluate(bs.x, bs.a, bs._mask(False), False, bs.stream.cuda_stream)
+            bs.stream.synchronize()
     fn()
     torch.cuda.synchronize()
     fn()
@@ -102,7 +108,7 @@ def main(argv=None):
                     help="for rocprofv3 --pmc runs: one launch of the kernel named (cloud: "
                          "the headline pass and one 256 MiB copy; ring: velocity + "
                          "stretching + PSE on the ring of 131072; adjoint: its adjoint; "
-                         "batch: one stage-1 launch over the 96 sets)")
+                         "batch: a stage-1 and a later-stage launch over the 96 sets)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -803,37 +803,45 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     constexpr int kUnrollFar = std::is_same<Real, float>::value ? (kUnroll == 8 ? 16 : kUnroll) : 2;
     constexpr int kChunks = TILE / SUB;
     constexpr unsigned kAllFar = (1u << kChunks) - 1u;
-    static_assert(kChunks <= 32, "far bits");
+    static_assert(kChunks <= 32 && 64 % kChunks == 0, "far bits");
     // Chunks of this tile in the far form (kFarS): their box keeps a gap of
     // more than kFarGapS radii from the targets' box.  Returns the
     // workgroup's bits (workgroup-uniform: a tile with a near chunk is
     // staged) and sets the wave's (its own, smaller box: a superset of the
     // workgroup's bits; wave-uniform).  Chunks at or past hi count as far
-    // and are skipped.
+    // and are skipped.  The tests run for a window of 64 chunks at a time,
+    // lane l testing chunk win + l, and two ballots hold the window's bits
+    // (tiles never straddle a window: 64 is a multiple of kChunks and the
+    // first window starts at the first tile).
+    int win = 0;
+    bool have_win = false;
+    uint64_t wg_far = 0, wv_far = 0;
     auto far_bits = [&](int t0, unsigned* wave_bits) {
-      unsigned bits = 0u, wbits = 0u;
-      auto gap2 = [&](const double* cb, const double* cen, const double* half) {
-        const double gx = fmax(fabs(cb[0] - cen[0]) * scale - half[0] - cb[3], 0.0),
-                     gy = fmax(fabs(cb[1] - cen[1]) * scale - half[1] - cb[4], 0.0),
-                     gz = fmax(fabs(cb[2] - cen[2]) * scale - half[2] - cb[5], 0.0);
-        return gx * gx + gy * gy + gz * gz;
-      };
-      const double wgc[3] = {cx, cy, cz};
-      const double wgh[3] = {half_ext[0], half_ext[1], half_ext[2]};
-#pragma unroll
-      for (int c = 0; c < kChunks; ++c) {
-        const int j0 = t0 + c * SUB;
+      const int c0 = t0 / SUB;
+      if (!have_win || c0 >= win + 64) {
+        auto gap2 = [&](const double* cb, const double* cen, const double* half) {
+          const double gx = fmax(fabs(cb[0] - cen[0]) * scale - half[0] - cb[3], 0.0),
+                       gy = fmax(fabs(cb[1] - cen[1]) * scale - half[1] - cb[4], 0.0),
+                       gz = fmax(fabs(cb[2] - cen[2]) * scale - half[2] - cb[5], 0.0);
+          return gx * gx + gy * gy + gz * gz;
+        };
+        const double wgc[3] = {cx, cy, cz};
+        const double wgh[3] = {half_ext[0], half_ext[1], half_ext[2]};
+        win = c0;
+        have_win = true;
+        const int64_t cc = static_cast<int64_t>(win) + (tid & 63);
         bool f = true, fw = true;
-        if (j0 < hi) {
-          const double* cb = far_box + static_cast<int64_t>(j0 / SUB) * 6;
+        if (cc * SUB < hi) {
+          const double* cb = far_box + cc * 6;
           f = gap2(cb, wgc, wgh) > kFarGapS * kFarGapS;
           fw = f || gap2(cb, &wbox[tid >> 6][0], &wbox[tid >> 6][3]) > kFarGapS * kFarGapS;
         }
-        bits |= f ? (1u << c) : 0u;
-        wbits |= fw ? (1u << c) : 0u;
+        wg_far = __ballot(f);
+        wv_far = __ballot(fw);
       }
-      *wave_bits = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(wbits)));
-      return static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bits)));
+      const int sh = c0 - win;
+      *wave_bits = static_cast<unsigned>((wv_far >> sh) & kAllFar);
+      return static_cast<unsigned>((wg_far >> sh) & kAllFar);
     };
     // One far chunk [j0, j0 + SUB) from the far records, in scalar loads.
     auto far_chunk = [&](int j0) {
@@ -916,8 +924,13 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     for (int t0 = lo; t0 < hi; t0 += TILE) {
       unsigned farbits = 0u, wfarbits = 0u;
       if constexpr (kFarS) farbits = far_bits(t0, &wfarbits);
-      // a tile whose chunks are all far needs no staging (and no barrier)
+      // A tile whose chunks are all far needs no staging, but its barrier
+      // stays: it keeps the workgroup's waves within a tile of each other, so
+      // that they read the same far records in the same stretch of time (the
+      // scalar cache then serves most of them; random cloud of 2^20: 632 ->
+      // 622 ms on one machine).
       bool far = false;
+      if (kFarS && farbits == kAllFar) __syncthreads();
       if (!kFarS || farbits != kAllFar) {
         __syncthreads();
         Real bmn[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
@@ -1250,7 +1263,10 @@ constexpr int min_waves() {
   // 512 lanes with far records: six waves per SIMD (three workgroups fit the
   // LDS of 512-source tiles), eight with 256-source tiles
   if (BLOCK == 512) return (!S::vort && !S::ens) ? (FARS ? (TILE <= 256 ? 8 : 6) : 4) : 1;
-  // PSE and LES sets of 256 lanes: three workgroups per CU
+  // velocity + stretching + PSE with 256 lanes: five waves per SIMD (96
+  // VGPRs, a few spills outside the pair loops; ring of 131,072: 17.6 ->
+  // 16.9 ms against four); the LES sets of 256 lanes: three workgroups per CU
+  if (BLOCK == 256 && S::pse && !S::vort && !S::ens) return 5;
   if (BLOCK == 256) return (S::pse || S::vort || S::ens) ? VPM_W256 : 1;
   return 1;
 }

@@ -129,7 +129,7 @@ _FP32 = {"SQ_INSTS_VALU_FMA_F32": 2, "SQ_INSTS_VALU_MUL_F32": 1, "SQ_INSTS_VALU_
 _FP64 = {"SQ_INSTS_VALU_FMA_F64": 2, "SQ_INSTS_VALU_MUL_F64": 1, "SQ_INSTS_VALU_ADD_F64": 1}
 
 
-def valu_summary(vals, pairs, flops_per_pair, kernel_ms, cus=256, peak_tflops=157.3):
+def valu_summary(vals, pairs, flops_per_pair, kernel_ms, cus=256, xcds=8, peak_tflops=157.3):
     """Executed-instruction figures of one launch from its counters: VALU
     instructions per pair (wave instructions x 64 lanes / pairs), FP32 and
     FP64 flops per pair by instruction class, executed FP32 TFLOP/s against
@@ -151,13 +151,20 @@ def valu_summary(vals, pairs, flops_per_pair, kernel_ms, cus=256, peak_tflops=15
         out["exec_fp64_flop_per_pair"] = f64 / pairs
     out["alg_tflops"] = flops_per_pair * pairs / (kernel_ms * 1e-3) / 1e12
     out["alg_frac"] = out["alg_tflops"] / peak_tflops
+    # GPU cycles of the dispatch: GRBM_GUI_ACTIVE sums the busy cycles of the
+    # eight XCDs (it reads 8 x 2.37e9 per second of kernel time here); without
+    # it, the kernel time at the nominal 2.4 GHz
+    if "GRBM_GUI_ACTIVE" in vals and vals["GRBM_GUI_ACTIVE"]:
+        cycles = vals["GRBM_GUI_ACTIVE"] / xcds
+        out["clock_ghz"] = cycles / (kernel_ms * 1e-3) / 1e9
+    else:
+        cycles = kernel_ms * 1e-3 * 2.4e9
     if "SQ_INSTS_VALU" in vals:
-        clk = kernel_ms * 1e-3 * 2.4e9 * cus
-        out["valu_wave_instr_per_clk_per_cu"] = vals["SQ_INSTS_VALU"] / clk
+        # wave64 VALU instructions issued per clock per CU (the FMA-chain probe
+        # peaks at 1.44 with packed FP32 off; 2.0 is the 157.3 TFLOP/s peak)
+        out["valu_wave_instr_per_clk_per_cu"] = vals["SQ_INSTS_VALU"] / (cycles * cus)
     if "SQ_ACTIVE_INST_VALU" in vals and "SQ_ACTIVE_INST_VALU2" in vals and vals["SQ_ACTIVE_INST_VALU"]:
         out["dual_issue_share"] = vals["SQ_ACTIVE_INST_VALU2"] / vals["SQ_ACTIVE_INST_VALU"]
-    if "SQ_ACTIVE_INST_VALU" in vals and "GRBM_GUI_ACTIVE" in vals and vals["GRBM_GUI_ACTIVE"]:
-        out["valu_busy"] = vals["SQ_ACTIVE_INST_VALU"] / cus / vals["GRBM_GUI_ACTIVE"]
     if "SQ_WAIT_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
         out["wait_any_share"] = vals["SQ_WAIT_ANY"] / vals["SQ_WAVE_CYCLES"]
     if "SQ_WAIT_INST_ANY" in vals and "SQ_WAVE_CYCLES" in vals and vals["SQ_WAVE_CYCLES"]:
@@ -166,7 +173,11 @@ def valu_summary(vals, pairs, flops_per_pair, kernel_ms, cus=256, peak_tflops=15
 
 
 def valu_markdown(dbs, kernel, workload, pairs, flops_per_pair, kernel_ms, title, command):
-    vals = kernel_counters(dbs, kernel)
+    return render_valu(kernel_counters(dbs, kernel), kernel, workload, pairs, flops_per_pair,
+                       kernel_ms, title, command)
+
+
+def render_valu(vals, kernel, workload, pairs, flops_per_pair, kernel_ms, title, command):
     sm = valu_summary(vals, pairs, flops_per_pair, kernel_ms)
     rec = {"workload": workload, "kernel": kernel, **sm}
     lines = [f"# {title}", "", "Commands (from the repository root, on one MI355X):", "",
@@ -237,12 +248,21 @@ def main(argv=None):
     v.add_argument("--workload", required=True)
     v.add_argument("--pairs", type=float, required=True)
     v.add_argument("--flops-per-pair", type=float, required=True)
-    v.add_argument("--kernel-ms", type=float, required=True)
+    g = v.add_mutually_exclusive_group(required=True)
+    g.add_argument("--kernel-ms", type=float)
+    g.add_argument("--kernel-ms-from", metavar="STATS_DB",
+                   help="take the kernel's average duration from a --kernel-trace database")
     v.add_argument("--title", required=True)
     v.add_argument("--command", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "valu":
-        print(valu_markdown(a.dbs, a.kernel, a.workload, a.pairs, a.flops_per_pair, a.kernel_ms,
+        ms = a.kernel_ms
+        if ms is None:
+            hits = [avg for k, _, _, avg, _ in kernel_stats(a.kernel_ms_from) if a.kernel in k]
+            if not hits:
+                raise SystemExit(f"kernel {a.kernel!r} not in {a.kernel_ms_from}")
+            ms = hits[-1] / 1000.0
+        print(valu_markdown(a.dbs, a.kernel, a.workload, a.pairs, a.flops_per_pair, ms,
                             a.title, a.command), end="")
         return
     if a.cmd == "stats":
