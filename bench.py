@@ -38,6 +38,7 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 # cross product, the alpha_j sum for alpha_i x G and the f2 (alpha_i.c) r
 # term); Kahan and FP64 folds are not counted.
 FLOPS_PER_PAIR_VEL_STRETCH = 53
+FLOPS_PER_PAIR_VEL_STRETCH_PSE = 61
 
 
 def traffic_workload(n):
@@ -366,6 +367,46 @@ def main(argv=None):
             "adjoint_over_forward": times["adjoint"] / times["forward"]}
         log(f"[bench] adjoint/forward = {times['adjoint']:.2f}/{times['forward']:.2f} ms "
             f"= {times['adjoint'] / times['forward']:.2f}")
+
+        # the ring's fused pass alone: prepared once, HIP events around the
+        # run on the launch stream (the kernel and, for these 512 target
+        # blocks, the split reduction), as for the headline kernel
+        router = ("vel", "stretch", "pse")
+        ro = L.ForwardOpts(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE, L.PREC_F32C, L.SIGMA_UNIFORM,
+                           0, -1)
+        rpart = P._particles(rp, ra, rs, rn)
+        rbuf = [torch.empty((3, nr), dtype=torch.float64, device=dev) for _ in range(3)]
+        rfo = L.ForwardOut(nr, *[b.data_ptr() for b in rbuf], None, None)
+        rwsb = lib.vpm_forward_workspace_bytes(nr, C.byref(ro))
+        rws = torch.empty(rwsb, dtype=torch.uint8, device=dev)
+        L.call("vpm_forward_prepare", C.byref(rpart), C.byref(ro), rws.data_ptr(), rwsb, sh)
+        L.call("vpm_forward_run", C.byref(rpart), C.byref(ro), C.byref(rfo), rws.data_ptr(), rwsb,
+               sh)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(10):
+            L.call("vpm_forward_run", C.byref(rpart), C.byref(ro), C.byref(rfo), rws.data_ptr(),
+                   rwsb, sh)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ring_ms = e0.elapsed_time(e1) / 10
+        rpairs = float(nr) * float(nr)
+        rcfg = P.tile_configs()[P.tile_config_default(router)]
+        rkernel = "fused_kernel<float, 7u, true, {}, {}, {}, {}, {}>".format(*rcfg)
+        rach = FLOPS_PER_PAIR_VEL_STRETCH_PSE * rpairs / (ring_ms * 1e-3) / 1e12
+        rroof = {"bound": "valu", "achieved": rach, "peak": FP32_VECTOR_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": rach / FP32_VECTOR_PEAK_TFLOPS, "kernel": rkernel,
+                 "kernel_ms": ring_ms, "flops_per_pair": FLOPS_PER_PAIR_VEL_STRETCH_PSE}
+        rexec = profiling.load_record(f"ring N={nr}, velocity+stretching+PSE, f32c", rkernel,
+                                      "exec_fp32_flop_per_pair")
+        if rexec is not None:
+            rx = rexec * rpairs / (ring_ms * 1e-3) / 1e12
+            rroof["executed"] = {"flops_per_pair": rexec, "achieved": rx,
+                                 "frac": rx / FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        result["adjoint"]["forward_roofline"] = rroof
+        log(f"[bench] ring forward pass alone: {ring_ms:.2f} ms, {rach:.1f} TFLOP/s "
+            f"({rach / FP32_VECTOR_PEAK_TFLOPS:.3f} of peak)")
 
     # ---- batched persistent launch, graph-replayed RK4 steps -----------------
     if not args.skip_batch:
