@@ -1337,7 +1337,7 @@ struct TileCfg {
 constexpr TileCfg kCfgs[] = {
     {512, 1, 512, 64, 0},  {256, 1, 512, 64, 0},  {256, 2, 256, 32, 1}, {256, 1, 256, 64, 0},
     {256, 1, 256, 64, 1},  {512, 1, 512, 128, 0}, {512, 1, 1024, 64, 0}, {128, 1, 256, 64, 0},
-    {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},  {512, 1, 256, 128, 0},
+    {256, 2, 512, 64, 0},  {256, 2, 256, 32, 0},  {512, 1, 256, 128, 0}, {256, 1, 256, 128, 1},
 };
 
 constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
@@ -1346,17 +1346,18 @@ constexpr bool hot_mask(unsigned m) { return m == 3u || m == 7u || m == 27u; }
 //  * velocity + stretching, uniform sigma: config 5 (512 lanes x one target,
 //    128-source FP32 partials: half the folds of config 0; random cloud of
 //    2^20: 685 ms against 693 ms, rel-L2 5.0e-7 against 4.5e-7);
-//  * velocity + stretching + PSE, uniform sigma: config 4 (256 lanes,
-//    256-source tiles, partials folded straight into FP64; with the moment
-//    form on far tiles: ring of 131072 18.6 ms against 19.4 for config 1
-//    and 20.2 for config 0; cloud of 131072 16.9 against 17.0 and 16.3);
+//  * velocity + stretching + PSE, uniform sigma: config 11 (256 lanes,
+//    256-source tiles, 128-source chunks, partials folded straight into
+//    FP64; ring of 131072 with far records and five waves per SIMD: 16.1 ms
+//    against 16.4-16.5 for config 4, its 64-source-chunk twin, 17.0 with
+//    512-source tiles, 17.2 for config 3 and 18.6 for config 0);
 //  * other sets: config 1 (256 lanes) with PSE, vorticity or enstrophy --
 //    their kernels need more than 128 VGPRs, where a 512-lane workgroup would
 //    run alone on its CU -- and config 0 without.
 constexpr unsigned kHeavyOutputs = VPM_OUT_PSE | VPM_OUT_VORT | VPM_OUT_ENSTROPHY;
 constexpr int default_config(unsigned kernel_mask, bool unif) {
   return (unif && kernel_mask == 3u)      ? 5
-         : (unif && kernel_mask == 7u)    ? 4
+         : (unif && kernel_mask == 7u)    ? 11
          : (kernel_mask & kHeavyOutputs) ? 1
                                           : 0;
 }
@@ -1439,7 +1440,7 @@ int launch_f32(const FusedArgs& a, int cfg, hipStream_t st, int* nb) {
   if constexpr (UNIF && hot_mask(OUT)) {
     switch (cfg) {
       VPM_CFG(2) VPM_CFG(3) VPM_CFG(4) VPM_CFG(5) VPM_CFG(6) VPM_CFG(7) VPM_CFG(8) VPM_CFG(9)
-      VPM_CFG(10)
+      VPM_CFG(10) VPM_CFG(11)
       default:
         break;
     }
@@ -1848,7 +1849,7 @@ int launch_batch_mask(const BatchArgs& b, int prec, int cfg, int grid, hipStream
                             kCfgs[I].sub, kCfgs[I].comp>(b, grid, st);
   switch (cfg) {
     VPM_BCFG(1) VPM_BCFG(2) VPM_BCFG(3) VPM_BCFG(4) VPM_BCFG(5) VPM_BCFG(6) VPM_BCFG(7)
-    VPM_BCFG(8) VPM_BCFG(9) VPM_BCFG(10)
+    VPM_BCFG(8) VPM_BCFG(9) VPM_BCFG(10) VPM_BCFG(11)
     default:
       break;
   }

@@ -77,12 +77,14 @@ def test_argument_errors_return_status_without_launch():
 def test_tile_config_defaults():
     """tile_config = -1 picks one shape per output set: 512 lanes with
     128-source partials for velocity + stretching, 256 lanes with 256-source
-    tiles and direct FP64 folds for velocity + stretching + PSE, 256 lanes
-    for the other sets with PSE or the LES outputs; the batch uses the shape
-    of velocity + stretching + PSE with PSE or LES, 512 lanes without."""
+    tiles, 128-source chunks and direct FP64 folds for velocity + stretching
+    + PSE, 256 lanes for the other sets with PSE or the LES outputs; the
+    batch uses the 64-source-chunk twin of the PSE shape with PSE or LES,
+    512 lanes without."""
     lib = L.load()
+    assert lib.vpm_tile_config_count() == 12
     assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH) == 5
-    assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 4
+    assert lib.vpm_tile_config_default(L.OUT_VEL | L.OUT_STRETCH | L.OUT_PSE) == 11
     assert lib.vpm_tile_config_default(31) == 1
     assert lib.vpm_tile_config_default(27) == 1
     assert lib.vpm_tile_config_default(L.OUT_VEL) == 0

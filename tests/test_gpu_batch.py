@@ -97,7 +97,7 @@ def test_batch_coincident_particles(gpu, precision, tol):
         assert float(bs.enstrophy[k]) == pytest.approx(O.total_enstrophy(x, a, sg), rel=10 * tol)
 
 
-@pytest.mark.parametrize("cfg", range(10))
+@pytest.mark.parametrize("cfg", range(12))
 def test_batch_tile_configs(gpu, cfg):
     sets = small_sets(1)
     bs = B.BatchedSystem(sets, dt=0.01, precision="f32c", tile_config=cfg)

@@ -99,7 +99,7 @@ def test_nonuniform_sigma(gpu, precision, tol, n, seed):
     check(res, ref, tol, ("general", precision, n))
 
 
-@pytest.mark.parametrize("cfg", range(10))
+@pytest.mark.parametrize("cfg", range(12))
 def test_tile_configs(gpu, cfg):
     """Every autotune candidate produces the same answer on a cloud."""
     x, a, s, nu = FM.random_cloud(4096 + 77, seed=11)
