@@ -65,8 +65,8 @@ struct FusedArgs {
   int chunk;
   // Far-field records of the FP32 pass (far_records_kernel): per chunk of
   // SUB sorted sources its centre and scaled half-extent (far_box, 6 doubles)
-  // and per source FP32 chunk-centred offsets, strength, moment and (PSE)
-  // low parts (far_rec, FarRec<OUT>::kStride floats).  Null where unused.
+  // and per source FP32 chunk-centred offsets, strength and moment
+  // (far_rec, FarRec<OUT>::kStride floats).  Null where unused.
   const float* far_rec;
   const double* far_box;
 };
@@ -396,8 +396,8 @@ struct Mom {  // a source's moment a_j x x_j
 
 // Far-field record of one source in the FP32 pass, relative to the centre of
 // its chunk of SUB sorted sources and scaled by 1/sigma: offset (3),
-// strength (3), moment a_j x x_j (3) and, with PSE, the strength's low parts
-// a_j - fp32(a_j) (3).  A wave reads the records of a far chunk through
+// strength (3) and moment a_j x x_j (3); the far form of PSE needs no low
+// parts of the strengths (pair_moment).  A wave reads the records of a far chunk through
 // scalar loads (every lane pairs with the same source), so they reach the
 // pair instructions as scalar operands with no LDS staging, no LDS reads and
 // no barrier; the target's offset to the chunk centre, and a_i x x_i, are
@@ -408,10 +408,16 @@ constexpr int kMinFarChunk = 64, kMaxFarChunk = 128;  // SUB of the shapes that 
 #define VPM_FAR_GROUP 4
 #endif
 constexpr int kFarGroup = VPM_FAR_GROUP;  // far records per scalar-load group
+// Floats per far record: 9, padded to 12 in the sets with PSE, whose
+// kernels ran faster with groups of four records on whole 64-byte lines
+// (ring of 131,072: 16.0 against 16.3 ms; the 96-set batch: 723 against
+// 743 ms per step); velocity + stretching ran faster unpadded (618 against
+// 624 ms at N = 2^20, same machine).
 template <unsigned OUT>
 struct FarRec {
   static constexpr int kStride = (OUT & VPM_OUT_PSE) ? 12 : 9;
 };
+constexpr int kMaxFarStride = 12;
 
 // Constant-address-space view of the records: a load through it with a
 // wave-uniform address becomes a scalar (s_load) load.
@@ -474,11 +480,7 @@ __device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, 
     r[6] = static_cast<float>(g.ay * z - g.az * y);
     r[7] = static_cast<float>(g.az * x - g.ax * z);
     r[8] = static_cast<float>(g.ax * y - g.ay * x);
-    if (stride == 12) {
-      r[9] = static_cast<float>(g.ax - static_cast<double>(ax));
-      r[10] = static_cast<float>(g.ay - static_cast<double>(ay));
-      r[11] = static_cast<float>(g.az - static_cast<double>(az));
-    }
+    for (int q = 9; q < stride; ++q) r[q] = 0.0f;
   } else {
     r[0] = r[1] = r[2] = 1e18f;
     for (int q = 3; q < stride; ++q) r[q] = 0.0f;
@@ -885,12 +887,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           sr.ay = f[4];
           sr.az = f[5];
           const Mom<Real> m{f[6], f[7], f[8]};
-          Real slo[3] = {Real(0), Real(0), Real(0)};
-          if constexpr (RS == 12) {
-            slo[0] = f[9];
-            slo[1] = f[10];
-            slo[2] = f[11];
-          }
+          const Real slo[3] = {Real(0), Real(0), Real(0)};  // unused by the far form
 #pragma unroll
           for (int r = 0; r < R; ++r) pair_moment<Real, OUT, UNIF>(part[r], tc[r], pc[r], sr, m, slo, kc);
         };
@@ -1555,7 +1552,7 @@ int layout(int64_t n, const vpm_forward_opts_t* o, WsLayout* L, bool need_cub = 
   const int64_t nn = n > 0 ? n : 1;
   // + one group of records read ahead past the last chunk
   L->far_rec = take(far ? (static_cast<size_t>(ceil_div(nn, kMaxFarChunk)) * kMaxFarChunk +
-                           kFarGroup) * 12 * sizeof(float)
+                           kFarGroup) * kMaxFarStride * sizeof(float)
                         : 0);
   L->far_box = take(far ? static_cast<size_t>(ceil_div(nn, kMinFarChunk)) * 6 * sizeof(double) : 0);
   size_t cub_bytes = 0;
@@ -1901,7 +1898,7 @@ int batch_layout(int64_t n, int nsets, int n_items, BatchLayout* L, bool need_cu
   // chunk per set, plus one group read ahead past the last
   const size_t S = static_cast<size_t>(nsets > 0 ? nsets : 1);
   L->set_chunk0 = take((S + 1) * sizeof(int));
-  L->far_rec = take((N + kMaxFarChunk * S + kFarGroup) * 12 * sizeof(float));
+  L->far_rec = take((N + kMaxFarChunk * S + kFarGroup) * kMaxFarStride * sizeof(float));
   L->far_box = take((N / kMinFarChunk + S + 1) * 6 * sizeof(double));
   size_t cb = 0;
   if (need_cub) {
