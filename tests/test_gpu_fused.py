@@ -404,3 +404,25 @@ def test_single_pass_crosses_near_and_far_tiles(gpu, precision, tol, outputs, la
     res = P.forward(*to_dev(x, a, s, nu), outputs=outputs, precision=precision,
                     sigma_mode="uniform", max_splits=1)
     check(res, {k: ref[k] for k in outputs}, tol, ("single-pass", layout, precision))
+
+
+@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse")])
+def test_far_test_windows_and_partial_chunks(gpu, outputs):
+    """The far tests run per window of 64 chunks (one lane per chunk, two
+    ballots).  A cloud of 3 * 64 * 128 + 77 sources spans four windows of
+    the 128-source chunks (the last chunk partial) and, at the 64-source
+    chunks of the PSE shape, seven; split four ways its source ranges start
+    inside windows.  Every target of the FP32 pass, unsplit and split,
+    against the FP64 pass (rel-L2 <= 1e-6)."""
+    n = 3 * 64 * 128 + 77
+    x, a, s, nu = FM.random_cloud(n, seed=19)
+    nu = np.full(n, 1e-3)
+    print(f"[far-windows] n={n} {outputs}", flush=True)
+    d = to_dev(x, a, s, nu)
+    ref = P.forward(*d, outputs=outputs, precision="f64", sigma_mode="uniform")
+    for splits in (1, 4):
+        res = P.forward(*d, outputs=outputs, precision="f32c", sigma_mode="uniform",
+                        max_splits=splits)
+        for k in outputs:
+            err = rel_l2(res[k].cpu().numpy(), ref[k].cpu().numpy())
+            assert err <= F32C_TOL, (splits, k, err)
