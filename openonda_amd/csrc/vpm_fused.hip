@@ -1160,17 +1160,22 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           fold_sub(std::false_type{});
         }
       }
-      if constexpr (kComp) {
+    }
+    // The Kahan pairs carry the whole sweep: (sum, compensation) keeps the
+    // sum of the sub-tile partials to about twice FP32's precision, so they
+    // reach the FP64 totals once, here.  (Folded per tile, three totals the
+    // allocator keeps in scratch were reloaded and stored for every tile:
+    // 3e10 bytes written per launch at N = 2^20.)
+    if constexpr (kComp) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+      for (int r = 0; r < R; ++r)
 #pragma unroll
-          for (int q = 0; q < kNAcc; ++q) {
-            if (!S::used(q)) continue;
-            tot[r][q] += static_cast<double>(ksum[r][q]) - static_cast<double>(kcmp[r][q]);
-            ksum[r][q] = Real(0);
-            kcmp[r][q] = Real(0);
-          }
-      }
+        for (int q = 0; q < kNAcc; ++q) {
+          if (!S::used(q)) continue;
+          tot[r][q] += static_cast<double>(ksum[r][q]) - static_cast<double>(kcmp[r][q]);
+          ksum[r][q] = Real(0);
+          kcmp[r][q] = Real(0);
+        }
     }
   };
   if (g64)  // constant unless kBothG
