@@ -64,7 +64,7 @@ struct FusedArgs {
   int64_t pld;
   int chunk;
   // Far-field records of the FP32 pass (far_records_kernel): per chunk of
-  // SUB sorted sources its centre and scaled half-extent (far_box, 6 doubles)
+  // SUB sorted sources its centre and half-extent in radius units (far_box, 6 doubles)
   // and per source FP32 chunk-centred offsets, strength and moment
   // (far_rec, FarRec<OUT>::kStride floats).  Null where unused.
   const float* far_rec;
@@ -462,7 +462,7 @@ __device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, 
       hi = fmax(hi, red[3 + q][w]);
     }
     cen[q] = 0.5 * (lo + hi);
-    box[q] = cen[q];
+    box[q] = cen[q] * scale;  // centre and half-extent in radius units
     box[3 + q] = 0.5 * (hi - lo) * scale;
   }
   __syncthreads();
@@ -668,7 +668,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   if constexpr (kBothG && std::is_same<Real, float>::value)
     coh = a.g64_mode == VPM_G64_ALWAYS ||
           (a.g64_mode == VPM_G64_AUTO && coherent_wave<R>(tg, valid));
-  // each wave's own target box (centre, scaled half-extent) in wbox: far
+  // each wave's own target box (centre and half-extent in radius units) in wbox: far
   // chunks are decided per wave against it, the staging of a tile per
   // workgroup
   {
@@ -680,7 +680,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
         red[c][wave] = lo;
         red[3 + c][wave] = hi;
         if constexpr (kFarS) {  // wbox has one row per wave only then
-          wbox[wave][c] = 0.5 * (lo + hi);
+          wbox[wave][c] = 0.5 * (lo + hi) * scale;
           wbox[wave][3 + c] = 0.5 * (hi - lo) * scale;
         }
       }
@@ -732,9 +732,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
     if constexpr (kMoment) {
       const double x = (tg[r].x - cx) * scale, y = (tg[r].y - cy) * scale,
                    z = (tg[r].z - cz) * scale;
-      lds_x64[r * BLOCK + tid][0] = x;
-      lds_x64[r * BLOCK + tid][1] = y;
-      lds_x64[r * BLOCK + tid][2] = z;
+      // far records: the target's position in radius units (a far chunk's
+      // centre is subtracted from it); otherwise its offset to the block
+      // centre
+      lds_x64[r * BLOCK + tid][0] = kFarS ? tg[r].x * scale : x;
+      lds_x64[r * BLOCK + tid][1] = kFarS ? tg[r].y * scale : y;
+      lds_x64[r * BLOCK + tid][2] = kFarS ? tg[r].z * scale : z;
       pm[r][0] = static_cast<Real>(tg[r].ay * z - tg[r].az * y);
       pm[r][1] = static_cast<Real>(tg[r].az * x - tg[r].ax * z);
       pm[r][2] = static_cast<Real>(tg[r].ax * y - tg[r].ay * x);
@@ -822,12 +825,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       const int c0 = t0 / SUB;
       if (!have_win || c0 >= win + 64) {
         auto gap2 = [&](const double* cb, const double* cen, const double* half) {
-          const double gx = fmax(fabs(cb[0] - cen[0]) * scale - half[0] - cb[3], 0.0),
-                       gy = fmax(fabs(cb[1] - cen[1]) * scale - half[1] - cb[4], 0.0),
-                       gz = fmax(fabs(cb[2] - cen[2]) * scale - half[2] - cb[5], 0.0);
+          const double gx = fmax(fabs(cb[0] - cen[0]) - half[0] - cb[3], 0.0),
+                       gy = fmax(fabs(cb[1] - cen[1]) - half[1] - cb[4], 0.0),
+                       gz = fmax(fabs(cb[2] - cen[2]) - half[2] - cb[5], 0.0);
           return gx * gx + gy * gy + gz * gz;
         };
-        const double wgc[3] = {cx, cy, cz};
+        const double wgc[3] = {cx * scale, cy * scale, cz * scale};  // radius units
         const double wgh[3] = {half_ext[0], half_ext[1], half_ext[2]};
         win = c0;
         have_win = true;
@@ -850,8 +853,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
       if constexpr (kFarS) {
         constexpr int RS = FarRec<OUT>::kStride;
         const double* cb = far_box + static_cast<int64_t>(j0 / SUB) * 6;
-        const double ox = (cx - cb[0]) * scale, oy = (cy - cb[1]) * scale,
-                     oz = (cz - cb[2]) * scale;
+        const double ox = -cb[0], oy = -cb[1], oz = -cb[2];  // positions in radius units
         Tgt<Real> tc[R];
         Real pc[R][3];
 #pragma unroll
