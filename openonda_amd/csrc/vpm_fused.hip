@@ -201,7 +201,7 @@ __device__ __forceinline__ double store_target(const FusedArgs& a, const Src8& t
 // of their instruction on gfx950.
 template <typename Real>
 struct PairConst {
-  Real c15, c75, c22, c3;
+  Real c15, c75, c22, c3, c25;
 };
 
 // slo: the source's low parts (lx, ly, lz) when the PSE difference is taken
@@ -416,6 +416,9 @@ constexpr int kFarGroup = VPM_FAR_GROUP;  // far records per scalar-load group
 template <unsigned OUT>
 struct FarRec {
   static constexpr int kStride = (OUT & VPM_OUT_PSE) ? 12 : 9;
+  // Sets with stretching and PSE store 3 m_j (pair_moment, E3), the others m_j.
+  static constexpr bool kE3 = (OUT & VPM_OUT_PSE) && (OUT & VPM_OUT_STRETCH);
+  static constexpr double kMomScale = kE3 ? 3.0 : 1.0;
 };
 constexpr int kMaxFarStride = 12;
 
@@ -430,7 +433,7 @@ typedef __attribute__((address_space(4))) const float far_float_t;
 // chunk's own records and box.
 template <int CH>
 __device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, int n, int c,
-                                                  int stride, double sigma0,
+                                                  int stride, double mscale, double sigma0,
                                                   float* __restrict__ rec,
                                                   double* __restrict__ box) {
   static_assert(CH % 64 == 0 && CH <= 256, "chunk");
@@ -477,9 +480,9 @@ __device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, 
     r[3] = ax;
     r[4] = ay;
     r[5] = az;
-    r[6] = static_cast<float>(g.ay * z - g.az * y);
-    r[7] = static_cast<float>(g.az * x - g.ax * z);
-    r[8] = static_cast<float>(g.ax * y - g.ay * x);
+    r[6] = static_cast<float>(mscale * (g.ay * z - g.az * y));
+    r[7] = static_cast<float>(mscale * (g.az * x - g.ax * z));
+    r[8] = static_cast<float>(mscale * (g.ax * y - g.ay * x));
     for (int q = 9; q < stride; ++q) r[q] = 0.0f;
   } else {
     r[0] = r[1] = r[2] = 1e18f;
@@ -490,14 +493,21 @@ __device__ __forceinline__ void far_records_chunk(const Src8* __restrict__ src, 
 // The far records of one set: workgroup c builds chunk c.
 template <int CH>
 __global__ __launch_bounds__(CH) void far_records_kernel(const Src8* __restrict__ src, int n,
-                                                         int stride, float* __restrict__ rec,
+                                                         int stride, double mscale,
+                                                         float* __restrict__ rec,
                                                          double* __restrict__ box) {
   const int c = blockIdx.x;
-  far_records_chunk<CH>(src, n, c, stride, src[0].s, rec + static_cast<int64_t>(c) * CH * stride,
+  far_records_chunk<CH>(src, n, c, stride, mscale, src[0].s,
+                        rec + static_cast<int64_t>(c) * CH * stride,
                         box + static_cast<int64_t>(c) * 6);
 }
 
-template <typename Real, unsigned OUT, bool UNIF>
+// E3: p and the moment m come scaled by 3 (the far records of sets with
+// stretching and PSE hold 3 m_j; their H is divided by 3 in the FP64 fold),
+// so e arrives as 3 e and the weight t5 (3 + 7.5 t2) e is (t5 + 2.5 t7) 3e;
+// t7 = t5 t2 is then shared with PSE's t9 = t7 t2 and the vorticity's weight
+// (one instruction fewer per pair with PSE, two with PSE and vorticity).
+template <typename Real, unsigned OUT, bool UNIF, bool E3 = false>
 __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const Real* p,
                                             const SrcR<Real, UNIF>& s, const Mom<Real>& m,
                                             const Real* slo, const PairConst<Real>& k) {
@@ -522,12 +532,17 @@ __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const
   e = M::fma(t.ax, m.x, e);
   e = M::fma(t.ay, m.y, e);
   e = M::fma(t.az, m.z, e);
-  const Real w = t5 * M::fma(k.c75, t2, k.c3) * e;
+  const Real t7 = t5 * t2;
+  Real w;
+  if constexpr (E3)
+    w = M::fma(k.c25, t7, t5) * e;
+  else
+    w = t5 * M::fma(k.c75, t2, k.c3) * e;
   acc[kSd + 0] = M::fma(w, dx, acc[kSd + 0]);
   acc[kSd + 1] = M::fma(w, dy, acc[kSd + 1]);
   acc[kSd + 2] = M::fma(w, dz, acc[kSd + 2]);
   if constexpr (S::vort) {  // uniform radius: t^7; r^2 > 1e-8 always holds on a far tile
-    const Real wv = t5 * t2;
+    const Real wv = t7;
     acc[kW + 0] = M::fma(wv, s.ax, acc[kW + 0]);
     acc[kW + 1] = M::fma(wv, s.ay, acc[kW + 1]);
     acc[kW + 2] = M::fma(wv, s.az, acc[kW + 2]);
@@ -544,7 +559,7 @@ __device__ __forceinline__ void pair_moment(Real* acc, const Tgt<Real>& t, const
     // pair is more than 4 radii apart, so t9 = A^-9/2 < 17^-9/2 = 2.9e-6: both
     // sums are that small beside the near tiles' terms of the same target,
     // and so are their roundings -- the near tiles keep the differences.
-    const Real t9 = t5 * t2 * t2;
+    const Real t9 = t7 * t2;
     acc[kP + 0] = M::fma(t9, s.ax, acc[kP + 0]);
     acc[kP + 1] = M::fma(t9, s.ay, acc[kP + 1]);
     acc[kP + 2] = M::fma(t9, s.az, acc[kP + 2]);
@@ -770,9 +785,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
 #pragma unroll
   for (int r = 0; r < R; ++r) gsum[r][0] = gsum[r][1] = gsum[r][2] = 0.0;
 
-  PairConst<Real> kc{Real(1.5), Real(7.5), Real(2.2), Real(3)};
+  PairConst<Real> kc{Real(1.5), Real(7.5), Real(2.2), Real(3), Real(2.5)};
   if constexpr (std::is_same<Real, float>::value)  // keep them in VGPRs
     asm volatile("" : "+v"(kc.c15), "+v"(kc.c75), "+v"(kc.c22), "+v"(kc.c3));
+  // Far chunks of the sets with PSE or vorticity take the weight from t7
+  // (pair_moment, E3).
+  constexpr bool kE3 = kFarS && FarRec<OUT>::kE3;
 
   // ---- sweep the source tiles ------------------------------------------
   // One instantiation per G mode (G64: FP64 per-pair G, coherent waves), so
@@ -865,8 +883,15 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           pc[r][0] = tc[r].ay * tc[r].z - tc[r].az * tc[r].y;
           pc[r][1] = tc[r].az * tc[r].x - tc[r].ax * tc[r].z;
           pc[r][2] = tc[r].ax * tc[r].y - tc[r].ay * tc[r].x;
+          if constexpr (kE3) {  // with the records' 3 m_j, e arrives as 3 e
+            pc[r][0] *= kc.c3;
+            pc[r][1] *= kc.c3;
+            pc[r][2] *= kc.c3;
+          }
         }
         const far_float_t* rec = (const far_float_t*)(far_rec + static_cast<int64_t>(j0) * RS);
+        PairConst<Real> kf = kc;  // 2.5 in a register only while the chunk runs
+        if constexpr (kE3) asm volatile("" : "+v"(kf.c25));
         // Sources in groups of kFarGroup, two register buffers: the loads of
         // the next group issue after the first pair of the current one (whose
         // wait covers only the current group's loads), so they overlap the
@@ -891,7 +916,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           const Mom<Real> m{f[6], f[7], f[8]};
           const Real slo[3] = {Real(0), Real(0), Real(0)};  // unused by the far form
 #pragma unroll
-          for (int r = 0; r < R; ++r) pair_moment<Real, OUT, UNIF>(part[r], tc[r], pc[r], sr, m, slo, kc);
+          for (int r = 0; r < R; ++r)
+            pair_moment<Real, OUT, UNIF, kE3>(part[r], tc[r], pc[r], sr, m, slo, kf);
         };
         auto group = [&](const float* b, float* nb, int qn) {
           pair_of(b);
@@ -912,9 +938,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
           const double X = lds_x64[r * BLOCK + tid][0] + ox, Y = lds_x64[r * BLOCK + tid][1] + oy,
                        Z = lds_x64[r * BLOCK + tid][2] + oz;
           const double gx = part[r][kG + 0], gy = part[r][kG + 1], gz = part[r][kG + 2];
-          tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]);
-          tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]);
-          tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]);
+          constexpr double hs = kE3 ? 1.0 / 3.0 : 1.0;  // H of the records' 3 m_j
+          tot[r][kU + 0] += (Y * gz - Z * gy) + static_cast<double>(part[r][kH + 0]) * hs;
+          tot[r][kU + 1] += (Z * gx - X * gz) + static_cast<double>(part[r][kH + 1]) * hs;
+          tot[r][kU + 2] += (X * gy - Y * gx) + static_cast<double>(part[r][kH + 2]) * hs;
           part[r][kH + 0] = part[r][kH + 1] = part[r][kH + 2] = Real(0);
           far_pse_fold<Real, OUT>(part[r], t[r]);
         }
@@ -1416,7 +1443,8 @@ int launch_cfg(const FusedArgs& a0, hipStream_t st, int* nblocks) {
     static_assert(SUB == kMinFarChunk || SUB == kMaxFarChunk, "far chunk");
     hipLaunchKernelGGL(far_records_kernel<SUB>, dim3(static_cast<unsigned>(ceil_div(a.n, SUB))),
                        dim3(SUB), 0, st, a.src, a.n, FarRec<OUT>::kStride,
-                       const_cast<float*>(a.far_rec), const_cast<double*>(a.far_box));
+                       FarRec<OUT>::kMomScale, const_cast<float*>(a.far_rec),
+                       const_cast<double*>(a.far_box));
     VPM_LAUNCH_CHECK("far_records_kernel");
   }
   hipLaunchKernelGGL(kern, dim3(blocks, ks), dim3(BLOCK), 0, st, a);
@@ -1659,6 +1687,7 @@ __global__ __launch_bounds__(CH) void set_far_records_kernel(const Src8* __restr
                                                              const int* __restrict__ off,
                                                              const int* __restrict__ chunk0,
                                                              int nsets, int stride,
+                                                             double mscale,
                                                              float* __restrict__ rec,
                                                              double* __restrict__ box) {
   const int g = blockIdx.x;
@@ -1672,8 +1701,8 @@ __global__ __launch_bounds__(CH) void set_far_records_kernel(const Src8* __restr
   const int s = lo;
   const int n = off[s + 1] - off[s];
   const int c = g - chunk0[s];
-  far_records_chunk<CH>(src + off[s], n, c, stride, src[off[s]].s, rec + (int64_t)g * CH * stride,
-                        box + (int64_t)g * 6);
+  far_records_chunk<CH>(src + off[s], n, c, stride, mscale, src[off[s]].s,
+                        rec + (int64_t)g * CH * stride, box + (int64_t)g * 6);
 }
 
 __global__ void counter_reset_kernel(int* c) {
@@ -1832,8 +1861,8 @@ int launch_batch_cfg(const BatchArgs& b, int grid, hipStream_t st) {
     const int64_t max_chunks = ceil_div(b.n_total, SUB) + b.nsets;  // bound; extra groups exit
     hipLaunchKernelGGL(set_far_records_kernel<SUB>, dim3(static_cast<unsigned>(max_chunks)),
                        dim3(SUB), 0, st, b.src, b.inst_off, b.set_chunk0, b.nsets,
-                       FarRec<OUT>::kStride, const_cast<float*>(b.f.far_rec),
-                       const_cast<double*>(b.f.far_box));
+                       FarRec<OUT>::kStride, FarRec<OUT>::kMomScale,
+                       const_cast<float*>(b.f.far_rec), const_cast<double*>(b.f.far_box));
     VPM_LAUNCH_CHECK("set_far_records_kernel");
   }
   hipLaunchKernelGGL((fused_batch_kernel<Real, OUT, true, BLOCK, R, TILE, SUB, COMP>), dim3(grid),
