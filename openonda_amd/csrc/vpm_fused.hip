@@ -396,7 +396,8 @@ struct Mom {  // a source's moment a_j x x_j
 
 // Far-field record of one source in the FP32 pass, relative to the centre of
 // its chunk of SUB sorted sources and scaled by 1/sigma: offset (3),
-// strength (3) and moment a_j x x_j (3); the far form of PSE needs no low
+// strength (3) and moment a_j x x_j (3; 3 a_j x x_j in the sets with
+// stretching and PSE, see pair_moment); the far form of PSE needs no low
 // parts of the strengths (pair_moment).  A wave reads the records of a far chunk through
 // scalar loads (every lane pairs with the same source), so they reach the
 // pair instructions as scalar operands with no LDS staging, no LDS reads and
@@ -788,8 +789,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const int n, con
   PairConst<Real> kc{Real(1.5), Real(7.5), Real(2.2), Real(3), Real(2.5)};
   if constexpr (std::is_same<Real, float>::value)  // keep them in VGPRs
     asm volatile("" : "+v"(kc.c15), "+v"(kc.c75), "+v"(kc.c22), "+v"(kc.c3));
-  // Far chunks of the sets with PSE or vorticity take the weight from t7
-  // (pair_moment, E3).
+  // Far chunks of the sets with stretching and PSE take the weight from t7
+  // (pair_moment, E3; their records hold 3 m_j).
   constexpr bool kE3 = kFarS && FarRec<OUT>::kE3;
 
   // ---- sweep the source tiles ------------------------------------------
