@@ -406,14 +406,16 @@ def test_single_pass_crosses_near_and_far_tiles(gpu, precision, tol, outputs, la
     check(res, {k: ref[k] for k in outputs}, tol, ("single-pass", layout, precision))
 
 
-@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse")])
+@pytest.mark.parametrize("outputs", [("vel", "stretch"), ("vel", "stretch", "pse"), ALL])
 def test_far_test_windows_and_partial_chunks(gpu, outputs):
     """The far tests run per window of 64 chunks (one lane per chunk, two
     ballots).  A cloud of 3 * 64 * 128 + 77 sources spans four windows of
     the 128-source chunks (the last chunk partial) and, at the 64-source
     chunks of the PSE shape, seven; split four ways its source ranges start
     inside windows.  Every target of the FP32 pass, unsplit and split,
-    against the FP64 pass (rel-L2 <= 1e-6)."""
+    against the FP64 pass (rel-L2 <= 1e-6); the output sets with PSE take
+    the far weight from t^7 and records holding 3 m_j, all five outputs also
+    the vorticity's weight."""
     n = 3 * 64 * 128 + 77
     x, a, s, nu = FM.random_cloud(n, seed=19)
     nu = np.full(n, 1e-3)
